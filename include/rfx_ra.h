@@ -1,0 +1,344 @@
+/*
+ * rfx_ra.h - C-ABI of librfx_hip.so, the MI355X (gfx950) relational-algebra
+ * operator library that replaces Red Fox's precompiled binary kernels.
+ *
+ * Two layers are exported:
+ *
+ *  1. Native entry points rfx_*(): status-returning (0 = ok, negative =
+ *     RFX_E*), explicit context (HIP stream + cached workspace), element
+ *     counts instead of byte sizes, output sizes written to device memory
+ *     without a host round trip.  They are safe to issue back to back on
+ *     one stream and to capture into a HIP graph once the context's
+ *     workspace has been reserved.
+ *
+ *  2. Harmony-convention entry points named after the reference's binary
+ *     kernel families (set_b40csortpair, mgpu_join_find_bounds, ...).  They
+ *     take the same operands in the same order as the extern "C" wrappers
+ *     that RelationalAlgebraKernel.cpp generates (byte sizes, u64 type
+ *     codes, u64* size variables in device memory), run on the default
+ *     context, and are synchronous.  Where the reference baked the tuple
+ *     types into each generated instance, one trailing operand carries
+ *     them.  On failure they print the error and call abort(), as the
+ *     reference's check() does (ModernGPUJoin.cu:99-107).
+ *
+ * rfx_harmony_call() is the packed-u64 dispatcher that stands in for
+ * Harmony's JIT'd _Z_thunk_<name> wrappers (Runtime.cpp:1408-1817).
+ *
+ * Element widths are 1, 2, 4, 8 or 16 bytes.  16-byte elements are two
+ * little-endian u64 words, word 1 most significant; all key comparisons
+ * are unsigned, high word first.
+ */
+#ifndef RFX_RA_H
+#define RFX_RA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Status codes                                                           */
+/* ---------------------------------------------------------------------- */
+#define RFX_OK 0
+#define RFX_EINVAL -1   /* bad argument (width, null pointer, op code)   */
+#define RFX_ENOMEM -2   /* device allocation failed                      */
+#define RFX_EHIP -3     /* a HIP runtime call failed                     */
+#define RFX_ERANGE -4   /* size beyond the 32-bit index range            */
+#define RFX_ENOTSUP -5  /* combination not supported                     */
+#define RFX_EBUDGET -6  /* admission refused: over the HBM budget        */
+
+/* Type codes of nvcc::RelationalAlgebraKernel::DataType
+ * (redfox/redfox/nvcc/interface/RelationalAlgebraKernel.h:81-95). */
+#define RFX_I8 1
+#define RFX_I16 2
+#define RFX_I32 3
+#define RFX_I64 4
+#define RFX_I128 5
+#define RFX_F32 9
+#define RFX_F64 10
+
+typedef struct rfx_ctx rfx_ctx;
+
+/* ---------------------------------------------------------------------- */
+/* Context, memory, timing (plumbing)                                     */
+/* ---------------------------------------------------------------------- */
+const char* rfx_strerror(int status);
+/* Text of the last error on the calling thread ("" if none). */
+const char* rfx_last_error(void);
+/* Library version string. */
+const char* rfx_version(void);
+
+/* Create a context on `device` issuing work to `stream` (a hipStream_t;
+ * NULL = the null stream). */
+int rfx_ctx_create(rfx_ctx** ctx, int device, void* stream);
+int rfx_ctx_destroy(rfx_ctx* ctx);
+int rfx_ctx_set_stream(rfx_ctx* ctx, void* stream);
+void* rfx_ctx_stream(rfx_ctx* ctx);
+/* Make sure the cached workspace holds at least `bytes`. */
+int rfx_ctx_reserve(rfx_ctx* ctx, uint64_t bytes);
+uint64_t rfx_ctx_workspace_bytes(rfx_ctx* ctx);
+int rfx_ctx_synchronize(rfx_ctx* ctx);
+/* The process-wide context used by the Harmony-convention entry points. */
+rfx_ctx* rfx_default_ctx(void);
+
+int rfx_device_count(int* count);
+int rfx_set_device(int device);
+int rfx_malloc(void** ptr, uint64_t bytes);
+int rfx_free(void* ptr);
+int rfx_memcpy_htod(void* dst, const void* src, uint64_t bytes);
+int rfx_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+int rfx_memcpy_dtod(void* dst, const void* src, uint64_t bytes);
+int rfx_memset(void* dst, int value, uint64_t bytes);
+int rfx_device_synchronize(void);
+int rfx_mem_info(uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* Events on the context's stream; elapsed time in milliseconds. */
+int rfx_event_create(void** ev);
+int rfx_event_destroy(void* ev);
+int rfx_event_record(void* ev, rfx_ctx* ctx);
+int rfx_event_elapsed(float* ms, void* start, void* stop);
+
+/* Device-side synthetic data: splitmix64 stream value i is
+ * mix(seed + (i + 1) * 0x9E3779B97F4A7C15).  out[i] = (u32)(v >> 32). */
+int rfx_gen_u32(rfx_ctx* ctx, uint32_t* out, uint64_t n, uint64_t seed);
+/* out[i] = full 64-bit splitmix64 value. */
+int rfx_gen_u64(rfx_ctx* ctx, uint64_t* out, uint64_t n, uint64_t seed);
+/* out[i] = i (u32 row index). */
+int rfx_gen_iota_u32(rfx_ctx* ctx, uint32_t* out, uint64_t n);
+
+/* ---------------------------------------------------------------------- */
+/* Native operators                                                       */
+/* ---------------------------------------------------------------------- */
+
+/* Stable LSD radix sort (8-bit digits, onesweep with decoupled look-back)
+ * of n keys and, if val_bytes > 0, values, ordered by the low key_bits
+ * bits of each key.  The result lands in keys/vals.  Replaces
+ * redfox::sort_pair/sort_key (b40cSort.cu:58-1021) and the mergesort
+ * variants (ModernGPUSort.cu:238-574, with key_bits = 8*key_bytes). */
+int rfx_sort_pairs(rfx_ctx* ctx, void* keys, void* vals, uint64_t n,
+                   int key_bytes, int key_bits, int val_bytes);
+/* Workspace bytes rfx_sort_pairs needs for this shape. */
+uint64_t rfx_sort_workspace(uint64_t n, int key_bytes, int key_bits,
+                            int val_bytes);
+
+/* Sorted search of a sorted left key column in a sorted right key column
+ * (find_bounds_{8,16,32,64,128}, ModernGPUJoin.cu:109-663):
+ *   lower_bound[i] = #{r < left[i]},
+ *   left_count[i]  = exclusive prefix sum of #{r == left[i]},
+ *   *total (device u64) = number of matching pairs. */
+int rfx_join_find_bounds(rfx_ctx* ctx, int32_t* lower_bound,
+                         int32_t* left_count, uint64_t* total,
+                         const void* left, uint64_t nl, const void* right,
+                         uint64_t nr, int key_bytes);
+/* Load-balanced expansion (redfox::join, ModernGPUJoin.cu:665-733):
+ * for g < total, left_idx[g] = i with left_count[i] <= g <
+ * left_count[i] + count[i], right_idx[g] = lower_bound[i] + g -
+ * left_count[i].  `total` is a device pointer (read on the device). */
+int rfx_join_main(rfx_ctx* ctx, int32_t* left_idx, int32_t* right_idx,
+                  const uint64_t* total, uint64_t total_cap,
+                  const int32_t* lower_bound, const int32_t* left_count,
+                  uint64_t nl);
+/* Gather of joined rows (gather_*, ModernGPUJoin.h:43-200):
+ *   out[g] = (lkey[li] << key_shift) | (lval[li] << lval_shift) | rval[ri]
+ * lval / rval may be NULL.  `total` is a device pointer. */
+int rfx_join_gather(rfx_ctx* ctx, void* out, int out_bytes, const void* lkey,
+                    int lkey_bytes, const void* lval, int lval_bytes,
+                    const void* rval, int rval_bytes, const int32_t* left_idx,
+                    const int32_t* right_idx, const uint64_t* total,
+                    uint64_t total_cap, uint32_t key_shift,
+                    uint32_t lval_shift);
+/* find_bounds + expansion + gather in one pass over the output, without
+ * materialising the index columns.  *total must already be known (from
+ * rfx_join_find_bounds). */
+int rfx_join_expand_gather(rfx_ctx* ctx, void* out, int out_bytes,
+                           const void* lkey, int lkey_bytes, const void* lval,
+                           int lval_bytes, const void* rval, int rval_bytes,
+                           const int32_t* lower_bound,
+                           const int32_t* left_count, uint64_t nl,
+                           const uint64_t* total, uint64_t total_cap,
+                           uint32_t key_shift, uint32_t lval_shift);
+
+/* Reduce-by-key over runs of equal adjacent keys (reduce_by_key in
+ * Reduce.cu:175-1532).  keys/vals hold *n rows (device u64 *n);
+ * out_keys/out_vals receive one row per run and *n_out (device u64) the
+ * run count.  op: RFX_AGG_SUM, RFX_AGG_MIN, RFX_AGG_MAX. */
+#define RFX_AGG_SUM 0
+#define RFX_AGG_MIN 1
+#define RFX_AGG_MAX 2
+int rfx_reduce_by_key_f64(rfx_ctx* ctx, void* out_keys, double* out_vals,
+                          uint64_t* n_out, const void* keys,
+                          const double* vals, const uint64_t* n,
+                          uint64_t n_cap, int key_bytes, int op);
+int rfx_reduce_by_key_u32(rfx_ctx* ctx, void* out_keys, uint32_t* out_vals,
+                          uint64_t* n_out, const void* keys,
+                          const uint32_t* vals, const uint64_t* n,
+                          uint64_t n_cap, int key_bytes);
+/* Count of each run (reduce_by_key of the generate() ones). */
+int rfx_count_by_key(rfx_ctx* ctx, void* out_keys, uint32_t* out_counts,
+                     uint64_t* n_out, const void* keys, const uint64_t* n,
+                     uint64_t n_cap, int key_bytes);
+/* Single reductions (Reduce.cu:976-1136): sum of f64, sum of u32, and the
+ * maximum of u64 bit patterns.  Results go to device memory. */
+int rfx_total_f64(rfx_ctx* ctx, double* result, const double* v, uint64_t n);
+int rfx_total_u32(rfx_ctx* ctx, uint32_t* result, const uint32_t* v,
+                  uint64_t n);
+int rfx_max_u64(rfx_ctx* ctx, uint64_t* result, const uint64_t* v,
+                uint64_t n);
+
+/* Set operations on sorted inputs with multiset semantics (Union.cu,
+ * Difference.cu, Unique.cu): union keeps max(m,n) copies of a value,
+ * difference max(m-n,0), intersection min(m,n).  *n_out (device u64)
+ * receives the output element count; `out` must hold na+nb (union) or
+ * na elements. */
+#define RFX_SET_UNION 0
+#define RFX_SET_DIFFERENCE 1
+#define RFX_SET_INTERSECTION 2
+int rfx_set_op(rfx_ctx* ctx, int op, void* out, uint64_t* n_out,
+               const void* a, uint64_t na, const void* b, uint64_t nb,
+               int elem_bytes);
+/* Drop adjacent duplicates (thrust::unique); in and out may alias only if
+ * equal. */
+int rfx_unique(rfx_ctx* ctx, void* out, uint64_t* n_out, const void* in,
+               uint64_t n, int elem_bytes);
+
+/* Stable filter of packed tuples (select_field_constant, Select.h:30-114,
+ * plus its scan and gather): keep rows whose field at [field_shift,
+ * field_shift + field_bits) compares true against `constant`.
+ * Ops: RFX_CMP_*.  *n_out (device u64) receives the kept count. */
+#define RFX_CMP_EQ 0
+#define RFX_CMP_NE 1
+#define RFX_CMP_LT 2
+#define RFX_CMP_LE 3
+#define RFX_CMP_GT 4
+#define RFX_CMP_GE 5
+#define RFX_CMP_SLT 6
+#define RFX_CMP_SLE 7
+#define RFX_CMP_SGT 8
+#define RFX_CMP_SGE 9
+#define RFX_CMP_DLT 10
+#define RFX_CMP_DLE 11
+#define RFX_CMP_DGT 12
+#define RFX_CMP_DGE 13
+int rfx_select(rfx_ctx* ctx, void* out, uint64_t* n_out, const void* in,
+               uint64_t n, int tuple_bytes, uint32_t field_shift,
+               uint32_t field_bits, int op, uint64_t constant);
+/* Field permutation (permute, Project.h:58-130). */
+int rfx_project(rfx_ctx* ctx, void* out, int out_bytes, const void* in,
+                int in_bytes, uint64_t n, int nfields,
+                const uint32_t* in_shift, const uint32_t* bits,
+                const uint32_t* out_shift);
+/* split (Split_Merge.h:303-330): key = in >> key_shift, value = low
+ * key_shift bits.  val may be NULL (split_key). */
+int rfx_split(rfx_ctx* ctx, void* key, int key_bytes, void* val,
+              int val_bytes, const void* in, int in_bytes, uint64_t n,
+              uint32_t key_shift);
+/* merge (Split_Merge.h:441-519): out = (key << key_shift) | value. */
+int rfx_merge(rfx_ctx* ctx, void* out, int out_bytes, const void* key,
+              int key_bytes, const void* val, int val_bytes, uint64_t n,
+              uint32_t key_shift);
+/* generate (Split_Merge.h:541-555): out[i] = 1. */
+int rfx_generate_ones(rfx_ctx* ctx, uint32_t* out, uint64_t n);
+
+/* Fused TPC-H Q1 filter-group-aggregate over columns (the plan of
+ * q1.man.cpp:959-2329 in one pass): rows with shipdate <= cutoff, grouped
+ * by g = returnflag*2 + linestatus (6 groups).  out (device, 6*8 f64):
+ * [g*8+0] sum_qty, +1 sum_base_price, +2 sum_disc_price, +3 sum_charge,
+ * +4 sum_disc, +5 count, +6 avg_qty, +7 avg_price.  avg_disc =
+ * sum_disc / count.  Summation order is fixed by n alone. */
+int rfx_q1(rfx_ctx* ctx, double* out, const int32_t* shipdate,
+           const uint8_t* returnflag, const uint8_t* linestatus,
+           const int32_t* qty, const double* price, const double* disc,
+           const double* tax, uint64_t n, int32_t cutoff);
+
+/* ---------------------------------------------------------------------- */
+/* Harmony-convention entry points (synchronous, default context).        */
+/* Operand order follows RelationalAlgebraKernel.cpp; "P" = device        */
+/* pointer slot, "S" = scalar u64 slot in Harmony's packed arguments.     */
+/* ---------------------------------------------------------------------- */
+
+/* set_b40csortpair<id>(K* keys P, V* vals P, u64 key_bytes S,
+ *   u64 key_bits S, u64 value_type S) - RelationalAlgebraKernel.cpp:2745-2772.
+ * Key width = Clamp(key_bits)/8 as in the generated Tuple_key. */
+void set_b40csortpair(void* keys, void* vals, uint64_t key_bytes,
+                      uint64_t key_bits, uint64_t value_type);
+/* set_b40csortkey<id>(K* keys P, u64 bytes S, u64 bits S) - :2774-2798. */
+void set_b40csortkey(void* keys, uint64_t bytes, uint64_t key_bits);
+/* set_mgpusortpair<id>(K*, V*, u64 key_bytes, u64 key_type,
+ *   u64 value_type) - :2658-2712. */
+void set_mgpusortpair(void* keys, void* vals, uint64_t key_bytes,
+                      uint64_t key_type, uint64_t value_type);
+/* set_mgpusortkey<id>(K*, u64 bytes, u64 type) - :2714-2743. */
+void set_mgpusortkey(void* keys, uint64_t bytes, uint64_t type);
+
+/* mgpu_join_find_bounds<id>(int* lower_bound P, int* left_count P,
+ *   u64* result_size P, K* left P, u64 left_bytes S, K* right P,
+ *   u64 right_bytes S) - :1210-1301; trailing key_type (the generated
+ *   instance's key tuple width: RFX_I8..RFX_I128).  *result_size
+ *   receives the match count, as ModernGPUJoin.cu:566-567 writes it. */
+void mgpu_join_find_bounds(int32_t* lower_bound, int32_t* left_count,
+                           uint64_t* result_size, void* left_key,
+                           uint64_t left_bytes, void* right_key,
+                           uint64_t right_bytes, uint64_t key_type);
+/* mgpu_join_main<id>(int* left_idx P, int* right_idx P, u64 idx_bytes S,
+ *   int* lower_bound P, int* left_count P, u64 lb_bytes S) - :1377-1397. */
+void mgpu_join_main(int32_t* left_idx, int32_t* right_idx,
+                    uint64_t idx_bytes, int32_t* lower_bound,
+                    int32_t* left_count, uint64_t lb_bytes);
+/* join_get_result_size<id>(u64* index_size, u64* result_size)
+ * - :1188-1208: index_size = total*4, result_size = total*out_bytes.
+ * Trailing out_bytes (sizeof the generated output tuple). */
+void join_get_result_size(uint64_t* index_size, uint64_t* result_size,
+                          uint64_t out_bytes);
+/* mgpu_join_gather<id>(Out* result, LK* left_key, int* left_index,
+ *   RK* right_key, int* right_index, u64* size, [LV* left_value],
+ *   [RV* right_value]) - :1101-1186, all device pointers; *size is the
+ * index byte size.  The generated instance's tuple shapes come in
+ * `desc`: bits 0-7 out_bytes, 8-15 lkey_bytes, 16-23 lval_bytes (0 =
+ * none), 24-31 rval_bytes (0 = none), 32-39 key_shift, 40-47 lval_shift. */
+void mgpu_join_gather(void* result, void* left_key, int32_t* left_index,
+                      void* right_key, int32_t* right_index, uint64_t* size,
+                      void* left_value, void* right_value, uint64_t desc);
+
+/* set_reduce<id> / set_count<id>(RK* result_key P, RV* result_value P,
+ *   K* key P, u64* key_size P, V* value P, u64* value_size P) -
+ *   :2800-3045.  Sizes are byte counts, in and out.  Trailing desc:
+ *   bits 0-7 key_bytes, 8-15 op (RFX_AGG_*), 16-23 value type code
+ *   (RFX_F64 or RFX_I32). */
+void set_reduce(void* result_key, void* result_value, void* key,
+                uint64_t* key_size, void* value, uint64_t* value_size,
+                uint64_t desc);
+void set_count(void* result_key, void* result_value, void* key,
+               uint64_t* key_size, void* value, uint64_t* value_size,
+               uint64_t key_bytes);
+/* set_single_reduce<id>(R* result P, T* begin P, u64 bytes S) -
+ *   :2910-2946.  Trailing kind: 0 total (f64), 1 count (u32 sum),
+ *   2 max (u64 bit patterns). */
+void set_single_reduce(void* result, void* begin, uint64_t bytes,
+                       uint64_t kind);
+/* set_union<id> / set_difference<id>(T* out P, u64* out_bytes P, T* a P,
+ *   u64 a_bytes S, T* b P, u64 b_bytes S, u64 type S) - :3077-3100,
+ *   :983-1007.  set_intersection follows the same convention (declared
+ *   but unimplemented in the reference, :919-922). */
+void set_union(void* result, uint64_t* size, void* left, uint64_t left_bytes,
+               void* right, uint64_t right_bytes, uint64_t type);
+void set_difference(void* result, uint64_t* size, void* left,
+                    uint64_t left_bytes, void* right, uint64_t right_bytes,
+                    uint64_t type);
+void set_intersection(void* result, uint64_t* size, void* left,
+                      uint64_t left_bytes, void* right, uint64_t right_bytes,
+                      uint64_t type);
+/* set_unique<id>(T* begin P, u64* bytes P, u64 type S) - :2948-2968. */
+void set_unique(void* begin, uint64_t* bytes, uint64_t type);
+
+/* Packed-argument dispatcher: classify `kernel_name` by family prefix the
+ * way BinaryKernelLaunch::execute does (Runtime.cpp:672-1038) and call the
+ * entry point with args[i] read as a pointer or a scalar.  Returns
+ * RFX_OK, or RFX_EINVAL for an unknown family / wrong argument count. */
+int rfx_harmony_call(const char* kernel_name, const uint64_t* args,
+                     uint32_t nargs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

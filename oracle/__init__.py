@@ -1,0 +1,215 @@
+"""ctypes binding of the CPU oracle (librfx_oracle.so) and of the reference's
+own tuple codec built into _ref/ (when it was built).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, and only as the checker or the reported CPU
+baseline.  Nothing in redfox_amd/ imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_int32, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "librfx_oracle.so")
+REF_TUPLE = os.path.join(HERE, "_ref", "libref_tuple.so")
+
+P, U64, U32, I = c_void_p, c_uint64, c_uint32, c_int
+
+_SIG = {
+    "rfxo_tuple_extract": (U64, [P, P, I, I]),
+    "rfxo_tuple_insert": (None, [P, P, I, I, U64]),
+    "rfxo_bits_after": (U32, [P, I, I]),
+    "rfxo_tuple_bytes": (U32, [P, I]),
+    "rfxo_sort_pairs": (I, [P, P, U64, I, I, I]),
+    "rfxo_join_find_bounds": (U64, [P, P, P, U64, P, U64, I]),
+    "rfxo_join_main": (None, [P, P, U64, P, P, U64]),
+    "rfxo_join_gather": (None, [P, I, P, I, P, I, P, I, P, P, U64, U32, U32]),
+    "rfxo_cpujoin": (U64, [P, P, U64, P, U64, P, U64, I]),
+    "rfxo_reduce_by_key_f64": (U64, [P, P, P, P, U64, I, I, I]),
+    "rfxo_reduce_by_key_u32": (U64, [P, P, P, P, U64, I]),
+    "rfxo_total_double": (c_double, [P, U64, I]),
+    "rfxo_count_u32": (U32, [P, U64]),
+    "rfxo_max_double_bits": (U64, [P, U64]),
+    "rfxo_set_union": (U64, [P, P, U64, P, U64, I]),
+    "rfxo_set_difference": (U64, [P, P, U64, P, U64, I]),
+    "rfxo_set_intersection": (U64, [P, P, U64, P, U64, I]),
+    "rfxo_unique": (U64, [P, U64, I]),
+    "rfxo_select": (U64, [P, P, U64, I, U32, U32, I, U64]),
+    "rfxo_project": (None, [P, I, P, I, U64, I, P, P, P]),
+    "rfxo_split": (None, [P, I, P, I, P, I, U64, U32]),
+    "rfxo_merge": (None, [P, I, P, I, P, I, U64, U32]),
+    "rfxo_q1": (None, [P, P, P, P, P, P, P, P, U64, c_int32, I]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise ImportError(f"{LIB} missing: run `make -C oracle`")
+        _lib = ctypes.CDLL(LIB)
+        for name, (res, args) in _SIG.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def ref_tuple():
+    """The reference's Tuple.h codec, or None if oracle/_ref was not built."""
+    if not os.path.exists(REF_TUPLE):
+        return None
+    L = ctypes.CDLL(REF_TUPLE)
+    L.ref_extract.restype = U64
+    L.ref_extract.argtypes = [I, P, I]
+    L.ref_insert.argtypes = [I, P, I, U64]
+    for n in ("ref_strip_values", "ref_strip_keys", "ref_restore_values"):
+        getattr(L, n).argtypes = [I, I, P, P]
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+# dtype for an element width; 16-byte elements are (lo, hi) u64 pairs.
+def dtype_of(width: int):
+    return {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64, 16: np.uint64}[width]
+
+
+def count_of(arr: np.ndarray, width: int) -> int:
+    return arr.nbytes // width
+
+
+# ---- wrappers returning new numpy arrays --------------------------------
+def sort_pairs(keys, vals, key_bytes, key_bits, val_bytes):
+    k = np.array(keys, copy=True)
+    v = None if vals is None else np.array(vals, copy=True)
+    n = count_of(k, key_bytes)
+    rc = lib().rfxo_sort_pairs(_p(k), _p(v), n, key_bytes, key_bits, val_bytes)
+    assert rc == 0
+    return k, v
+
+
+def join_find_bounds(left, right, key_bytes):
+    nl, nr = count_of(left, key_bytes), count_of(right, key_bytes)
+    lb = np.zeros(nl, dtype=np.int32)
+    lc = np.zeros(nl, dtype=np.int32)
+    total = lib().rfxo_join_find_bounds(_p(lb), _p(lc), _p(left), nl, _p(right), nr, key_bytes)
+    return lb, lc, int(total)
+
+
+def join_main(lb, lc, total):
+    li = np.zeros(total, dtype=np.int32)
+    ri = np.zeros(total, dtype=np.int32)
+    lib().rfxo_join_main(_p(li), _p(ri), total, _p(lb), _p(lc), len(lb))
+    return li, ri
+
+
+def join_gather(out_bytes, lkey, lkey_bytes, lval, lval_bytes, rval, rval_bytes, li, ri,
+                key_shift, lval_shift):
+    total = len(li)
+    out = np.zeros(total * out_bytes // dtype_of(out_bytes)().itemsize, dtype=dtype_of(out_bytes))
+    lib().rfxo_join_gather(_p(out), out_bytes, _p(lkey), lkey_bytes, _p(lval), lval_bytes,
+                           _p(rval), rval_bytes, _p(li), _p(ri), total, key_shift, lval_shift)
+    return out
+
+
+def cpujoin(a, b, key_bytes):
+    na, nb = count_of(a, key_bytes), count_of(b, key_bytes)
+    total = lib().rfxo_cpujoin(None, None, 0, _p(a), na, _p(b), nb, key_bytes)
+    ai = np.zeros(total, dtype=np.int32)
+    bi = np.zeros(total, dtype=np.int32)
+    lib().rfxo_cpujoin(_p(ai), _p(bi), total, _p(a), na, _p(b), nb, key_bytes)
+    return ai, bi
+
+
+def reduce_by_key_f64(keys, vals, key_bytes, op=0, compensated=1):
+    n = count_of(keys, key_bytes)
+    ok = np.zeros_like(keys)
+    ov = np.zeros(n, dtype=np.float64)
+    g = lib().rfxo_reduce_by_key_f64(_p(ok), _p(ov), _p(keys), _p(vals), n, key_bytes, op,
+                                     compensated)
+    words = g * key_bytes // keys.itemsize
+    return ok[:words], ov[:g]
+
+
+def reduce_by_key_u32(keys, vals, key_bytes):
+    n = count_of(keys, key_bytes)
+    ok = np.zeros_like(keys)
+    ov = np.zeros(n, dtype=np.uint32)
+    g = lib().rfxo_reduce_by_key_u32(_p(ok), _p(ov), _p(keys), _p(vals), n, key_bytes)
+    return ok[:g * key_bytes // keys.itemsize], ov[:g]
+
+
+def total_double(v, compensated=1):
+    return float(lib().rfxo_total_double(_p(v), len(v), compensated))
+
+
+def count_u32(v):
+    return int(lib().rfxo_count_u32(_p(v), len(v)))
+
+
+def max_double_bits(v):
+    return int(lib().rfxo_max_double_bits(_p(v), len(v)))
+
+
+def set_op(op, a, b, eb):
+    na, nb = count_of(a, eb), count_of(b, eb)
+    out = np.zeros((na + nb) * eb // a.itemsize if a.size or b.size else 0, dtype=a.dtype)
+    fn = {0: "rfxo_set_union", 1: "rfxo_set_difference", 2: "rfxo_set_intersection"}[op]
+    m = getattr(lib(), fn)(_p(out), _p(a), na, _p(b), nb, eb)
+    return out[:m * eb // a.itemsize]
+
+
+def unique(a, eb):
+    d = np.array(a, copy=True)
+    m = lib().rfxo_unique(_p(d), count_of(d, eb), eb)
+    return d[:m * eb // d.itemsize]
+
+
+def select(inp, tuple_bytes, field_shift, field_bits, op, constant):
+    n = count_of(inp, tuple_bytes)
+    out = np.zeros_like(inp)
+    m = lib().rfxo_select(_p(out), _p(inp), n, tuple_bytes, field_shift, field_bits, op,
+                          constant & 0xFFFFFFFFFFFFFFFF)
+    return out[:m * tuple_bytes // inp.itemsize]
+
+
+def project(out_bytes, inp, in_bytes, in_shift, bits, out_shift):
+    n = count_of(inp, in_bytes)
+    a = np.ascontiguousarray(in_shift, dtype=np.uint32)
+    b = np.ascontiguousarray(bits, dtype=np.uint32)
+    c = np.ascontiguousarray(out_shift, dtype=np.uint32)
+    out = np.zeros(n * out_bytes // dtype_of(out_bytes)().itemsize, dtype=dtype_of(out_bytes))
+    lib().rfxo_project(_p(out), out_bytes, _p(inp), in_bytes, n, len(a), _p(a), _p(b), _p(c))
+    return out
+
+
+def split(inp, in_bytes, key_bytes, val_bytes, key_shift):
+    n = count_of(inp, in_bytes)
+    k = np.zeros(n * key_bytes // dtype_of(key_bytes)().itemsize, dtype=dtype_of(key_bytes))
+    v = (np.zeros(n * val_bytes // dtype_of(val_bytes)().itemsize, dtype=dtype_of(val_bytes))
+         if val_bytes else None)
+    lib().rfxo_split(_p(k), key_bytes, _p(v), val_bytes, _p(inp), in_bytes, n, key_shift)
+    return k, v
+
+
+def merge(out_bytes, key, key_bytes, val, val_bytes, key_shift):
+    n = count_of(key, key_bytes)
+    out = np.zeros(n * out_bytes // dtype_of(out_bytes)().itemsize, dtype=dtype_of(out_bytes))
+    lib().rfxo_merge(_p(out), out_bytes, _p(key), key_bytes, _p(val), val_bytes, n, key_shift)
+    return out
+
+
+def q1(shipdate, flag, status, qty, price, disc, tax, cutoff, compensated=1):
+    out = np.zeros(48, dtype=np.float64)
+    lib().rfxo_q1(_p(out), _p(shipdate), _p(flag), _p(status), _p(qty), _p(price), _p(disc),
+                  _p(tax), len(shipdate), cutoff, compensated)
+    return out.reshape(6, 8)

@@ -1,0 +1,116 @@
+"""Loader for librfx_hip.so, the gfx950 relational-algebra operator library.
+
+The shared library is built in-tree (``make -C redfox_amd/csrc``) and loaded
+from this package directory.  There is no fallback: if the library is missing
+or the HIP runtime cannot be loaded, importing :mod:`redfox_amd` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_float, c_int, c_uint32, c_uint64, c_void_p
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librfx_hip.so")
+
+P = c_void_p
+U64 = c_uint64
+U32 = c_uint32
+I = c_int
+U64P = POINTER(c_uint64)
+
+# name -> (restype, argtypes).  Mirrors include/rfx_ra.h.
+SIGNATURES = {
+    "rfx_strerror": (c_char_p, [I]),
+    "rfx_last_error": (c_char_p, []),
+    "rfx_version": (c_char_p, []),
+    "rfx_ctx_create": (I, [POINTER(c_void_p), I, P]),
+    "rfx_ctx_destroy": (I, [P]),
+    "rfx_ctx_set_stream": (I, [P, P]),
+    "rfx_ctx_stream": (P, [P]),
+    "rfx_ctx_reserve": (I, [P, U64]),
+    "rfx_ctx_workspace_bytes": (U64, [P]),
+    "rfx_ctx_synchronize": (I, [P]),
+    "rfx_default_ctx": (P, []),
+    "rfx_device_count": (I, [POINTER(c_int)]),
+    "rfx_set_device": (I, [I]),
+    "rfx_malloc": (I, [POINTER(c_void_p), U64]),
+    "rfx_free": (I, [P]),
+    "rfx_memcpy_htod": (I, [P, P, U64]),
+    "rfx_memcpy_dtoh": (I, [P, P, U64]),
+    "rfx_memcpy_dtod": (I, [P, P, U64]),
+    "rfx_memset": (I, [P, I, U64]),
+    "rfx_device_synchronize": (I, []),
+    "rfx_mem_info": (I, [U64P, U64P]),
+    "rfx_event_create": (I, [POINTER(c_void_p)]),
+    "rfx_event_destroy": (I, [P]),
+    "rfx_event_record": (I, [P, P]),
+    "rfx_event_elapsed": (I, [POINTER(c_float), P, P]),
+    "rfx_gen_u32": (I, [P, P, U64, U64]),
+    "rfx_gen_u64": (I, [P, P, U64, U64]),
+    "rfx_gen_iota_u32": (I, [P, P, U64]),
+    "rfx_sort_pairs": (I, [P, P, P, U64, I, I, I]),
+    "rfx_sort_workspace": (U64, [U64, I, I, I]),
+    "rfx_join_find_bounds": (I, [P, P, P, P, P, U64, P, U64, I]),
+    "rfx_join_main": (I, [P, P, P, P, U64, P, P, U64]),
+    "rfx_join_gather": (I, [P, P, I, P, I, P, I, P, I, P, P, P, U64, U32, U32]),
+    "rfx_join_expand_gather": (I, [P, P, I, P, I, P, I, P, I, P, P, U64, P, U64, U32, U32]),
+    "rfx_reduce_by_key_f64": (I, [P, P, P, P, P, P, P, U64, I, I]),
+    "rfx_reduce_by_key_u32": (I, [P, P, P, P, P, P, P, U64, I]),
+    "rfx_count_by_key": (I, [P, P, P, P, P, P, U64, I]),
+    "rfx_total_f64": (I, [P, P, P, U64]),
+    "rfx_total_u32": (I, [P, P, P, U64]),
+    "rfx_max_u64": (I, [P, P, P, U64]),
+    "rfx_set_op": (I, [P, I, P, P, P, U64, P, U64, I]),
+    "rfx_unique": (I, [P, P, P, P, U64, I]),
+    "rfx_select": (I, [P, P, P, P, U64, I, U32, U32, I, U64]),
+    "rfx_project": (I, [P, P, I, P, I, U64, I, P, P, P]),
+    "rfx_split": (I, [P, P, I, P, I, P, I, U64, U32]),
+    "rfx_merge": (I, [P, P, I, P, I, P, I, U64, U32]),
+    "rfx_generate_ones": (I, [P, P, U64]),
+    "rfx_q1": (I, [P, P, P, P, P, P, P, P, P, U64, c_int]),
+    "set_b40csortpair": (None, [P, P, U64, U64, U64]),
+    "set_b40csortkey": (None, [P, U64, U64]),
+    "set_mgpusortpair": (None, [P, P, U64, U64, U64]),
+    "set_mgpusortkey": (None, [P, U64, U64]),
+    "mgpu_join_find_bounds": (None, [P, P, P, P, U64, P, U64, U64]),
+    "mgpu_join_main": (None, [P, P, U64, P, P, U64]),
+    "join_get_result_size": (None, [P, P, U64]),
+    "mgpu_join_gather": (None, [P, P, P, P, P, P, P, P, U64]),
+    "set_reduce": (None, [P, P, P, P, P, P, U64]),
+    "set_count": (None, [P, P, P, P, P, P, U64]),
+    "set_single_reduce": (None, [P, P, U64, U64]),
+    "set_union": (None, [P, P, P, U64, P, U64, U64]),
+    "set_difference": (None, [P, P, P, U64, P, U64, U64]),
+    "set_intersection": (None, [P, P, P, U64, P, U64, U64]),
+    "set_unique": (None, [P, P, U64]),
+    "rfx_harmony_call": (I, [c_char_p, POINTER(c_uint64), U32]),
+}
+
+
+class RfxError(RuntimeError):
+    """A librfx_hip call returned a negative status."""
+
+    def __init__(self, fn: str, status: int, detail: str):
+        super().__init__(f"{fn} failed: status {status}: {detail}")
+        self.status = status
+
+
+def load(path: str = LIB_PATH, strict: bool = True) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build it with `make -C redfox_amd/csrc` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    missing = []
+    for name, (res, args) in SIGNATURES.items():
+        try:
+            fn = getattr(lib, name)
+        except AttributeError:
+            missing.append(name)
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    if missing and strict:
+        raise ImportError(f"{path} lacks symbols: {', '.join(missing)}")
+    return lib

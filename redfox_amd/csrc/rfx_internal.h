@@ -1,0 +1,97 @@
+// rfx_internal.h - host-side internals shared by the librfx_hip sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/rfx_ra.h"
+
+struct rfx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* ws = nullptr;         // cached device workspace
+  uint64_t ws_bytes = 0;
+  int device_cus = 256;       // compute units, for persistent grids
+};
+
+namespace rfx {
+
+// Record an error message for rfx_last_error() and return `code`.
+int fail(int code, const char* fmt, ...);
+
+// Check a HIP call; on failure record it and return RFX_EHIP.
+#define RFX_HIP(call)                                                   \
+  do {                                                                  \
+    hipError_t rfx_e_ = (call);                                         \
+    if (rfx_e_ != hipSuccess)                                           \
+      return ::rfx::fail(RFX_EHIP, "%s failed: %s (%s:%d)", #call,      \
+                         hipGetErrorString(rfx_e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define RFX_TRY(expr)            \
+  do {                           \
+    int rfx_s_ = (expr);         \
+    if (rfx_s_ != RFX_OK) return rfx_s_; \
+  } while (0)
+
+// Check the launch that was just issued.
+#define RFX_LAUNCHED() RFX_HIP(hipGetLastError())
+
+inline bool valid_width(int b) {
+  return b == 1 || b == 2 || b == 4 || b == 8 || b == 16;
+}
+
+inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+// Make sure ctx->ws holds `bytes` (synchronises the stream before growing).
+int ensure_workspace(rfx_ctx* ctx, uint64_t bytes);
+
+// A bump allocator over a workspace region.  Constructed with a null base
+// it only measures: `off` ends as the bytes the same sequence needs.
+struct Carve {
+  uintptr_t base;
+  uint64_t off = 0;
+  explicit Carve(void* b) : base(reinterpret_cast<uintptr_t>(b)) {}
+  template <class T>
+  T* take(uint64_t count) {
+    off = align_up(off, 256);
+    T* p = reinterpret_cast<T*>(base + off);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+// Zero `words` u32 words at `p` on the context stream with a fill kernel
+// (graph-capture safe, unlike hipMemsetAsync).
+int fill_u32(rfx_ctx* ctx, uint32_t* p, uint64_t words, uint32_t value);
+
+// Type-code helpers (RelationalAlgebraKernel.h:81-95).
+inline int bytes_of_type(uint64_t type) {
+  switch (type) {
+    case RFX_I8: return 1;
+    case RFX_I16: return 2;
+    case RFX_I32: return 4;
+    case RFX_I64: return 8;
+    case RFX_I128: return 16;
+    case RFX_F32: return 4;
+    case RFX_F64: return 8;
+    default: return 0;
+  }
+}
+// Clamp<bits>/8 (Tuple.h:30-37).
+inline int bytes_of_bits(uint64_t bits) {
+  if (bits <= 8) return 1;
+  if (bits <= 16) return 2;
+  if (bits <= 32) return 4;
+  if (bits <= 64) return 8;
+  if (bits <= 128) return 16;
+  return 0;
+}
+
+// Abort with the message, as the reference's check() does.
+[[noreturn]] void die(const char* where, int status);
+
+}  // namespace rfx

@@ -1,0 +1,388 @@
+// sort.hip - stable LSD radix sort for gfx950: 8-bit digits, one upfront
+// histogram pass over the keys for all digit positions, then one
+// "onesweep" kernel per digit that ranks a tile in registers/LDS, obtains
+// its global digit offsets by decoupled look-back and scatters through an
+// LDS reorder buffer.
+//
+// Replaces redfox::sort_pair / sort_key (b40cSort.cu:58-1021: b40c LSD
+// radix, 4-5 bit digits, upsweep + spine + downsweep per digit, pong
+// buffers cudaMalloc'd per call) and the mergesort variants
+// (ModernGPUSort.cu:238-574).  Both are stable, so for the same key bits
+// they produce exactly the order this sort produces.
+//
+// Everything after the launch is decided on the device: a digit position
+// whose histogram has one full bucket is an identity permutation and is
+// skipped; which of the two buffers holds the data is tracked in a device
+// word, and a final kernel copies back only if it is the pong buffer.  No
+// host round trip, so a sort can be captured into a HIP graph.
+#include "rfx_device.h"
+#include "rfx_internal.h"
+
+namespace rfx {
+namespace {
+
+constexpr int kBins = 256;
+
+template <int KB, int VB>
+struct SortTune {
+  static constexpr int kMax = KB > VB ? KB : VB;
+  static constexpr int NT = 256;
+  static constexpr int IPT = kMax >= 16 ? 8 : 16;
+  static constexpr int TILE = NT * IPT;
+};
+
+struct SortLayout {
+  uint32_t* hist;    // [P][256]
+  uint32_t* base;    // [P][256] exclusive global digit offsets
+  uint32_t* ctrs;    // [P][2]   tile ticket, finished tiles
+  uint32_t* skip;    // [P]
+  uint32_t* sel;     // [1]      0: data in caller's buffers, 1: in pong
+  uint32_t* status;  // [P][tiles][256]
+  void* pong_keys;
+  void* pong_vals;
+  uint64_t zero_words;  // words from hist to the end of status
+};
+
+static void carve_sort(Carve& c, uint64_t n, int kb, int vb, int passes, uint64_t tiles,
+                       SortLayout* L) {
+  // The zeroed block comes first so it is one contiguous range.
+  const uint64_t status_words = static_cast<uint64_t>(passes) * tiles * kBins;
+  L->hist = c.take<uint32_t>(static_cast<uint64_t>(passes) * kBins);
+  L->base = c.take<uint32_t>(static_cast<uint64_t>(passes) * kBins);
+  L->ctrs = c.take<uint32_t>(static_cast<uint64_t>(passes) * 2);
+  L->skip = c.take<uint32_t>(static_cast<uint64_t>(passes));
+  L->sel = c.take<uint32_t>(1);
+  L->status = c.take<uint32_t>(status_words);
+  L->zero_words = (c.off - (reinterpret_cast<uintptr_t>(L->hist) - c.base)) / 4;
+  L->pong_keys = c.take<char>(n * kb);
+  L->pong_vals = vb ? c.take<char>(n * vb) : nullptr;
+}
+
+__device__ __forceinline__ uint32_t pass_mask(int pass, uint32_t key_bits) {
+  uint32_t width = key_bits - 8u * static_cast<uint32_t>(pass);
+  if (width > 8) width = 8;
+  return (1u << width) - 1u;
+}
+
+// Histogram of every digit position in one read of the keys.
+template <int KB>
+__global__ __launch_bounds__(256) void k_radix_hist(const typename Elem<KB>::T* __restrict__ keys,
+                                                    uint64_t n, int passes, uint32_t key_bits,
+                                                    uint32_t* __restrict__ hist) {
+  using K = typename Elem<KB>::T;
+  __shared__ uint32_t lh[16 * kBins];
+  for (int i = threadIdx.x; i < passes * kBins; i += 256) lh[i] = 0;
+  __syncthreads();
+  uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
+    K k = keys[i];
+    for (int p = 0; p < passes; ++p) {
+      uint32_t d = digit_of(k, 8u * p, pass_mask(p, key_bits));
+      atomicAdd(&lh[p * kBins + d], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < passes * kBins; i += 256)
+    if (lh[i]) atomicAdd(&hist[i], lh[i]);
+}
+
+// Exclusive scan of each digit histogram; flags identity passes.
+__global__ __launch_bounds__(256) void k_radix_scan(const uint32_t* __restrict__ hist,
+                                                    uint32_t* __restrict__ base,
+                                                    uint32_t* __restrict__ skip, uint64_t n) {
+  __shared__ uint32_t tmp[8];
+  const int p = blockIdx.x;
+  const uint32_t c = hist[p * kBins + threadIdx.x];
+  uint32_t total;
+  uint32_t e = block_excl_scan<256>(c, tmp, &total);
+  base[p * kBins + threadIdx.x] = e;
+  if (c == n) skip[p] = 1u;  // every key has the same digit here
+}
+
+template <int KB, int VB, int NT, int IPT>
+__global__ __launch_bounds__(NT) void k_onesweep(
+    typename Elem<KB>::T* __restrict__ k0, typename Elem<KB>::T* __restrict__ k1,
+    typename Elem<VB>::T* __restrict__ v0, typename Elem<VB>::T* __restrict__ v1, uint64_t n,
+    int pass, uint32_t key_bits, const uint32_t* __restrict__ base, uint32_t* __restrict__ ctrs,
+    const uint32_t* __restrict__ skip, uint32_t* __restrict__ sel, uint32_t* __restrict__ status,
+    uint32_t ntiles) {
+  using K = typename Elem<KB>::T;
+  using V = typename Elem<VB>::T;
+  constexpr int NW = NT / kWave;
+  constexpr int TILE = NT * IPT;
+  constexpr int RB = (KB > VB ? KB : VB) * TILE;
+
+  if (skip[pass]) return;
+  const uint32_t cur = *sel;
+  const K* __restrict__ kin = cur ? k1 : k0;
+  K* __restrict__ kout = cur ? k0 : k1;
+  const V* __restrict__ vin = cur ? v1 : v0;
+  V* __restrict__ vout = cur ? v0 : v1;
+
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t wcnt[NW][kBins];
+  __shared__ uint32_t bstart[kBins];
+  __shared__ uint32_t gdelta[kBins];
+  __shared__ uint32_t scan_tmp[NW + 1];
+  __shared__ __attribute__((aligned(16))) unsigned char rbuf[RB];
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t w = tid / kWave;
+  const uint32_t shift = 8u * static_cast<uint32_t>(pass);
+  const uint32_t dmask = pass_mask(pass, key_bits);
+
+  if (tid == 0) s_tile = atomicAdd(&ctrs[pass * 2], 1u);
+  for (uint32_t i = tid; i < NW * kBins; i += NT) (&wcnt[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t tile_base = static_cast<uint64_t>(tile) * TILE;
+  const uint64_t wave_base = tile_base + static_cast<uint64_t>(w) * kWave * IPT;
+
+  K keys[IPT];
+  V vals[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+    if (idx < n) {
+      keys[j] = kin[idx];
+      if constexpr (VB > 0) vals[j] = vin[idx];
+    } else {
+      keys[j] = K{};
+      if constexpr (VB > 0) vals[j] = V{};
+    }
+  }
+
+  // Stable rank within the wave: lanes sharing a digit are found by one
+  // ballot per digit bit; the highest such lane bumps the wave counter.
+  uint32_t pos[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+    const bool valid = idx < n;
+    const uint32_t d = digit_of(keys[j], shift, dmask);
+    uint64_t m = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      if ((dmask >> b) & 1u) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = ballot(bit);
+        m &= bit ? bb : ~bb;
+      }
+    }
+    const uint32_t below = mbcnt(m);
+    const uint32_t cnt = __popcll(m);
+    uint32_t prior = 0;
+    if (valid) {
+      prior = wcnt[w][d];
+      if (below == cnt - 1) wcnt[w][d] = prior + cnt;
+    }
+    pos[j] = prior + below;
+  }
+  __syncthreads();
+
+  // Per-digit tile totals, wave prefixes, and the look-back.
+  uint32_t total_d = 0;
+  uint32_t* st = status + (static_cast<uint64_t>(pass) * ntiles) * kBins;
+  if (tid < kBins) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww) {
+      uint32_t c = wcnt[ww][tid];
+      wcnt[ww][tid] = run;
+      run += c;
+    }
+    total_d = run;
+    status_store(st + static_cast<uint64_t>(tile) * kBins + tid,
+                 (tile == 0 ? kFlagIncl : kFlagAgg) | total_d);
+  }
+  uint32_t all;
+  const uint32_t bs = block_excl_scan<NT>(tid < kBins ? total_d : 0u, scan_tmp, &all);
+  if (tid < kBins) {
+    bstart[tid] = bs;
+    uint32_t excl = 0;
+    if (tile > 0) {
+      excl = lookback(st, tile, kBins, tid);
+      status_store(st + static_cast<uint64_t>(tile) * kBins + tid, kFlagIncl | (excl + total_d));
+    }
+    gdelta[tid] = base[pass * kBins + tid] + excl - bs;
+  }
+  __syncthreads();
+
+  // Reorder through LDS so each digit's run is written contiguously.
+  K* kbuf = reinterpret_cast<K*>(rbuf);
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+    if (idx < n) {
+      const uint32_t d = digit_of(keys[j], shift, dmask);
+      pos[j] += bstart[d] + wcnt[w][d];
+      kbuf[pos[j]] = keys[j];
+    }
+  }
+  __syncthreads();
+  const uint32_t tile_n =
+      static_cast<uint32_t>((n - tile_base) < TILE ? (n - tile_base) : TILE);
+  uint32_t dst[IPT];
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    const uint32_t i = k * NT + tid;
+    if (i < tile_n) {
+      const K key = kbuf[i];
+      dst[k] = gdelta[digit_of(key, shift, dmask)] + i;
+      if (dst[k] < n) kout[dst[k]] = key;  // always true when ranks are right
+    }
+  }
+  if constexpr (VB > 0) {
+    __syncthreads();
+    V* vbuf = reinterpret_cast<V*>(rbuf);
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+      if (idx < n) vbuf[pos[j]] = vals[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      const uint32_t i = k * NT + tid;
+      if (i < tile_n && dst[k] < n) vout[dst[k]] = vbuf[i];
+    }
+  }
+
+  // The last tile to finish flips the buffer selector.  Every block read
+  // *sel before arriving here, so nobody reads the new value this pass.
+  if (tid == 0) {
+    uint32_t prev = atomicAdd(&ctrs[pass * 2 + 1], 1u);
+    if (prev == ntiles - 1)
+      __hip_atomic_store(sel, cur ^ 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <int KB, int VB>
+__global__ __launch_bounds__(256) void k_copyback(typename Elem<KB>::T* __restrict__ k0,
+                                                  const typename Elem<KB>::T* __restrict__ k1,
+                                                  typename Elem<VB>::T* __restrict__ v0,
+                                                  const typename Elem<VB>::T* __restrict__ v1,
+                                                  uint64_t n, const uint32_t* __restrict__ sel) {
+  if (*sel == 0) return;
+  uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
+    k0[i] = k1[i];
+    if constexpr (VB > 0) v0[i] = v1[i];
+  }
+}
+
+template <int KB, int VB>
+uint64_t sort_ws_bytes(uint64_t n, int key_bits) {
+  using Tn = SortTune<KB, VB>;
+  const int passes = (key_bits + 7) / 8;
+  const uint64_t tiles = (n + Tn::TILE - 1) / Tn::TILE;
+  Carve s(nullptr);
+  SortLayout L;
+  carve_sort(s, n, KB, VB, passes, tiles, &L);
+  return s.off;
+}
+
+template <int KB, int VB>
+int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
+  using K = typename Elem<KB>::T;
+  using V = typename Elem<VB>::T;
+  using Tn = SortTune<KB, VB>;
+  if (n < 2) return RFX_OK;
+  if (n > kCountMask) return fail(RFX_ERANGE, "sort: %llu rows exceeds 2^30-1",
+                                  static_cast<unsigned long long>(n));
+  const int passes = (key_bits + 7) / 8;
+  const uint64_t tiles = (n + Tn::TILE - 1) / Tn::TILE;
+  RFX_TRY(ensure_workspace(ctx, sort_ws_bytes<KB, VB>(n, key_bits)));
+  Carve c(ctx->ws);
+  SortLayout L;
+  carve_sort(c, n, KB, VB, passes, tiles, &L);
+  RFX_TRY(fill_u32(ctx, L.hist, L.zero_words, 0u));
+
+  uint64_t hb = (n + 256 * 16 - 1) / (256 * 16);
+  uint64_t hmax = static_cast<uint64_t>(ctx->device_cus) * 8;
+  if (hb > hmax) hb = hmax;
+  if (hb == 0) hb = 1;
+  k_radix_hist<KB><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
+      static_cast<const K*>(keys), n, passes, static_cast<uint32_t>(key_bits), L.hist);
+  RFX_LAUNCHED();
+  k_radix_scan<<<passes, 256, 0, ctx->stream>>>(L.hist, L.base, L.skip, n);
+  RFX_LAUNCHED();
+  for (int p = 0; p < passes; ++p) {
+    k_onesweep<KB, VB, Tn::NT, Tn::IPT><<<static_cast<uint32_t>(tiles), Tn::NT, 0, ctx->stream>>>(
+        static_cast<K*>(keys), static_cast<K*>(L.pong_keys), static_cast<V*>(vals),
+        static_cast<V*>(L.pong_vals), n, p, static_cast<uint32_t>(key_bits), L.base, L.ctrs,
+        L.skip, L.sel, L.status, static_cast<uint32_t>(tiles));
+    RFX_LAUNCHED();
+  }
+  uint64_t cb = (n + 255) / 256;
+  if (cb > 8192) cb = 8192;
+  k_copyback<KB, VB><<<static_cast<uint32_t>(cb), 256, 0, ctx->stream>>>(
+      static_cast<K*>(keys), static_cast<const K*>(L.pong_keys), static_cast<V*>(vals),
+      static_cast<const V*>(L.pong_vals), n, L.sel);
+  RFX_LAUNCHED();
+  return RFX_OK;
+}
+
+template <int KB>
+int sort_dispatch_v(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits, int vb) {
+  switch (vb) {
+    case 0: return sort_impl<KB, 0>(ctx, keys, nullptr, n, key_bits);
+    case 1: return sort_impl<KB, 1>(ctx, keys, vals, n, key_bits);
+    case 2: return sort_impl<KB, 2>(ctx, keys, vals, n, key_bits);
+    case 4: return sort_impl<KB, 4>(ctx, keys, vals, n, key_bits);
+    case 8: return sort_impl<KB, 8>(ctx, keys, vals, n, key_bits);
+    case 16: return sort_impl<KB, 16>(ctx, keys, vals, n, key_bits);
+    default: return fail(RFX_EINVAL, "sort: value width %d", vb);
+  }
+}
+
+template <int KB>
+uint64_t ws_dispatch_v(uint64_t n, int key_bits, int vb) {
+  switch (vb) {
+    case 0: return sort_ws_bytes<KB, 0>(n, key_bits);
+    case 1: return sort_ws_bytes<KB, 1>(n, key_bits);
+    case 2: return sort_ws_bytes<KB, 2>(n, key_bits);
+    case 4: return sort_ws_bytes<KB, 4>(n, key_bits);
+    case 8: return sort_ws_bytes<KB, 8>(n, key_bits);
+    case 16: return sort_ws_bytes<KB, 16>(n, key_bits);
+    default: return 0;
+  }
+}
+
+}  // namespace
+}  // namespace rfx
+
+using namespace rfx;
+
+extern "C" int rfx_sort_pairs(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bytes,
+                              int key_bits, int val_bytes) {
+  if (!ctx) return fail(RFX_EINVAL, "rfx_sort_pairs: null context");
+  if (!valid_width(key_bytes)) return fail(RFX_EINVAL, "rfx_sort_pairs: key width %d", key_bytes);
+  if (val_bytes != 0 && !valid_width(val_bytes))
+    return fail(RFX_EINVAL, "rfx_sort_pairs: value width %d", val_bytes);
+  if (key_bits <= 0 || key_bits > 8 * key_bytes)
+    return fail(RFX_EINVAL, "rfx_sort_pairs: key_bits %d for %d-byte keys", key_bits, key_bytes);
+  if (n && !keys) return fail(RFX_EINVAL, "rfx_sort_pairs: null keys");
+  if (n && val_bytes && !vals) return fail(RFX_EINVAL, "rfx_sort_pairs: null values");
+  switch (key_bytes) {
+    case 1: return sort_dispatch_v<1>(ctx, keys, vals, n, key_bits, val_bytes);
+    case 2: return sort_dispatch_v<2>(ctx, keys, vals, n, key_bits, val_bytes);
+    case 4: return sort_dispatch_v<4>(ctx, keys, vals, n, key_bits, val_bytes);
+    case 8: return sort_dispatch_v<8>(ctx, keys, vals, n, key_bits, val_bytes);
+    case 16: return sort_dispatch_v<16>(ctx, keys, vals, n, key_bits, val_bytes);
+  }
+  return fail(RFX_EINVAL, "rfx_sort_pairs: key width %d", key_bytes);
+}
+
+extern "C" uint64_t rfx_sort_workspace(uint64_t n, int key_bytes, int key_bits, int val_bytes) {
+  if (!valid_width(key_bytes) || key_bits <= 0 || key_bits > 8 * key_bytes) return 0;
+  switch (key_bytes) {
+    case 1: return ws_dispatch_v<1>(n, key_bits, val_bytes);
+    case 2: return ws_dispatch_v<2>(n, key_bits, val_bytes);
+    case 4: return ws_dispatch_v<4>(n, key_bits, val_bytes);
+    case 8: return ws_dispatch_v<8>(n, key_bits, val_bytes);
+    case 16: return ws_dispatch_v<16>(n, key_bits, val_bytes);
+  }
+  return 0;
+}
