@@ -23,11 +23,19 @@ namespace {
 
 constexpr int kBins = 256;
 
+#ifndef RFX_SORT_NT
+#define RFX_SORT_NT 512
+#endif
+#ifndef RFX_SORT_IPT
+#define RFX_SORT_IPT 16
+#endif
+// Tile shape: NT threads x IPT keys per thread; 16-byte elements halve IPT
+// to stay within the LDS reorder buffer and VGPR budget.
 template <int KB, int VB>
 struct SortTune {
   static constexpr int kMax = KB > VB ? KB : VB;
-  static constexpr int NT = 256;
-  static constexpr int IPT = kMax >= 16 ? 8 : 16;
+  static constexpr int NT = RFX_SORT_NT;
+  static constexpr int IPT = kMax >= 16 ? RFX_SORT_IPT / 2 : RFX_SORT_IPT;
   static constexpr int TILE = NT * IPT;
 };
 
@@ -64,23 +72,36 @@ __device__ __forceinline__ uint32_t pass_mask(int pass, uint32_t key_bits) {
   return (1u << width) - 1u;
 }
 
-// Histogram of every digit position in one read of the keys.
-template <int KB>
+// Histogram of every digit position in one read of the keys.  With an
+// aligned key pointer each thread reads 16 bytes at a time.
+template <int KB, bool VEC>
 __global__ __launch_bounds__(256) void k_radix_hist(const typename Elem<KB>::T* __restrict__ keys,
                                                     uint64_t n, int passes, uint32_t key_bits,
                                                     uint32_t* __restrict__ hist) {
   using K = typename Elem<KB>::T;
+  constexpr int PER = (VEC && KB < 16) ? 16 / KB : 1;
   __shared__ uint32_t lh[16 * kBins];
   for (int i = threadIdx.x; i < passes * kBins; i += 256) lh[i] = 0;
   __syncthreads();
-  uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
-  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
-    K k = keys[i];
-    for (int p = 0; p < passes; ++p) {
-      uint32_t d = digit_of(k, 8u * p, pass_mask(p, key_bits));
-      atomicAdd(&lh[p * kBins + d], 1u);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
+  const uint64_t gtid = blockIdx.x * 256ull + threadIdx.x;
+  auto count = [&](K k) {
+    for (int p = 0; p < passes; ++p)
+      atomicAdd(&lh[p * kBins + digit_of(k, 8u * p, pass_mask(p, key_bits))], 1u);
+  };
+  uint64_t done = 0;
+  if constexpr (PER > 1) {
+    struct alignas(16) Vec { K k[PER]; };
+    const Vec* kv = reinterpret_cast<const Vec*>(keys);
+    const uint64_t nvec = n / PER;
+    for (uint64_t v = gtid; v < nvec; v += stride) {
+      const Vec x = kv[v];
+#pragma unroll
+      for (int e = 0; e < PER; ++e) count(x.k[e]);
     }
+    done = nvec * PER;
   }
+  for (uint64_t i = done + gtid; i < n; i += stride) count(keys[i]);
   __syncthreads();
   for (int i = threadIdx.x; i < passes * kBins; i += 256)
     if (lh[i]) atomicAdd(&hist[i], lh[i]);
@@ -303,8 +324,12 @@ int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
   uint64_t hmax = static_cast<uint64_t>(ctx->device_cus) * 8;
   if (hb > hmax) hb = hmax;
   if (hb == 0) hb = 1;
-  k_radix_hist<KB><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
-      static_cast<const K*>(keys), n, passes, static_cast<uint32_t>(key_bits), L.hist);
+  if (reinterpret_cast<uintptr_t>(keys) % 16 == 0)
+    k_radix_hist<KB, true><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
+        static_cast<const K*>(keys), n, passes, static_cast<uint32_t>(key_bits), L.hist);
+  else
+    k_radix_hist<KB, false><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
+        static_cast<const K*>(keys), n, passes, static_cast<uint32_t>(key_bits), L.hist);
   RFX_LAUNCHED();
   k_radix_scan<<<passes, 256, 0, ctx->stream>>>(L.hist, L.base, L.skip, n);
   RFX_LAUNCHED();
