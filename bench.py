@@ -1,0 +1,293 @@
+#!/usr/bin/env python3
+"""Benchmark of the RA hot path on one MI355X.
+
+Headline (`value`): stable LSD radix sort of 2^28 (u32 key, u32 value) pairs,
+keys uniform over the full 32-bit range, values = row index - the sort the
+reference runs as set_b40csortpair (b40cSort.cu:752-780).  One step sorts one
+such batch that is already resident in HBM (each step has its own input, all
+generated before the timed region).  `roofline` covers the dominant kernel,
+the onesweep digit pass, timed per launch with HIP events on its stream
+during the timed steps.  `cpu_baseline` is the oracle's serial LSD sort on a
+bounded sample, on this host.  `ops` adds the other workloads: the u64
+Zipf(1.1) sort, the config-1 join and an HBM-scale join, and TPC-H Q1 over
+60M rows.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "rows/s + % HBM roofline per op (join, sort, agg); batched qps, p99 latency"
+SEED = 0x52464F58000000CC
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def median(xs):
+    return float(np.median(np.array(xs)))
+
+
+def roofline(bytes_per_launch, launches, total_ms, traffic=None):
+    avg_ms = total_ms / max(launches, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "launch_ms": round(avg_ms, 4), "launches": launches,
+            "bytes_per_launch": int(bytes_per_launch)}
+
+
+# ---------------------------------------------------------------------------
+def bench_sort_u32(rf, ctx, steps, warmup):
+    n = 1 << 28
+    total = steps + warmup
+    if total * 8 * n > 200 << 30:
+        raise SystemExit("bench.py: too many steps for distinct resident inputs (max 25)")
+    keys, vals = [], []
+    for i in range(total):
+        k = rf.DeviceBuffer(4 * n)
+        v = rf.DeviceBuffer(4 * n)
+        ctx.gen_u32(k, n, SEED + 1 + i)
+        ctx.iota_u32(v, n)
+        keys.append(k)
+        vals.append(v)
+    ctx.reserve(rf.lib().rfx_sort_workspace(n, 4, 32, 4))
+    for i in range(warmup):
+        ctx.sort_pairs(keys[i], vals[i], n, 4, 32, 4)
+    ctx.synchronize()
+    ctx.profile(True)
+    timer = rf.Timer(ctx)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    timer.begin()
+    for i in range(warmup, total):
+        ctx.sort_pairs(keys[i], vals[i], n, 4, 32, 4)
+    ev_ms = timer.end()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    launches, sweep_ms = ctx.profile_read("sort.onesweep")
+    hist = ctx.profile_read("sort.hist")
+    ctx.profile(False)
+    # spot check: the last output is sorted and stable
+    k = keys[-1].download(np.uint32, n)
+    v = vals[-1].download(np.uint32, n)
+    ok = bool(np.all(k[1:] >= k[:-1]) and np.all(v[1:][k[1:] == k[:-1]] > v[:-1][k[1:] == k[:-1]]))
+    for b in keys + vals:
+        b.close()
+    # Each onesweep launch reads and writes every (key, value): 16 B/row.
+    rl = roofline(16 * n, launches, sweep_ms)
+    return {"n": n, "wall_s": wall, "event_ms": ev_ms, "ms_per_step": 1e3 * wall / steps,
+            "rows_per_s": n * steps / wall, "roofline": rl, "verified_sorted": ok,
+            "hist_ms": hist[1] / max(hist[0], 1),
+            "pass_model_GBps": 68 * n * steps / wall / 1e9}
+
+
+def bench_sort_u64_zipf(rf, ctx, iters=3):
+    n = 1 << 27
+    k0 = rf.DeviceBuffer(8 * n)
+    v0 = rf.DeviceBuffer(4 * n)
+    ctx.gen_zipf_u64(k0, n, SEED + 2, 1.1, 1 << 32)
+    k = rf.DeviceBuffer(8 * n)
+    v = rf.DeviceBuffer(4 * n)
+    L = rf.lib()
+    times = []
+    ctx.profile(True)
+    for _ in range(iters + 1):
+        L.rfx_memcpy_dtod(k.ptr, k0.ptr, 8 * n)
+        ctx.iota_u32(v, n)
+        ctx.synchronize()
+        t = rf.Timer(ctx)
+        t.begin()
+        ctx.sort_pairs(k, v, n, 8, 64, 4)
+        times.append(t.end())
+        t.close()
+    launches, ms = ctx.profile_read("sort.onesweep")
+    ctx.profile(False)
+    ms_sort = median(times[1:])
+    kk = k.download(np.uint64, n)
+    ok = bool(np.all(kk[1:] >= kk[:-1]))
+    for b in (k0, v0, k, v):
+        b.close()
+    return {"n": n, "ms": ms_sort, "rows_per_s": n / (ms_sort * 1e-3),
+            "pass_model_GBps": 200 * n / (ms_sort * 1e-3) / 1e9,
+            "roofline": roofline(24 * n, launches, ms), "verified_sorted": ok}
+
+
+def bench_join(rf, ctx, n_r, n_s, key_bits, iters):
+    """Sort-merge join R x S -> (key, r_val, s_val) 16-byte rows.  'core'
+    times find_bounds + expand/gather on sorted inputs; 'plan' adds the two
+    sorts."""
+    from redfox_amd import datagen
+
+    rk, rv, sk, sv = datagen.join_tables(SEED, n_r, n_s, key_bits)
+    src = {k: rf.DeviceBuffer.from_numpy(a) for k, a in dict(rk=rk, rv=rv, sk=sk, sv=sv).items()}
+    work = {k: rf.DeviceBuffer(b.nbytes) for k, b in src.items()}
+    lb = rf.DeviceBuffer(4 * n_r)
+    lc = rf.DeviceBuffer(4 * n_r)
+    tot = rf.u64_cell()
+    L = rf.lib()
+
+    def reset():
+        for k in src:
+            L.rfx_memcpy_dtod(work[k].ptr, src[k].ptr, src[k].nbytes)
+
+    reset()
+    ctx.sort_pairs(work["rk"], work["rv"], n_r, 4, 32, 4)
+    ctx.sort_pairs(work["sk"], work["sv"], n_s, 4, 32, 4)
+    ctx.join_find_bounds(lb, lc, tot, work["rk"], n_r, work["sk"], n_s, 4)
+    ctx.synchronize()
+    total = rf.read_u64(tot)
+    out = rf.DeviceBuffer(16 * max(total, 1))
+    plan, core = [], []
+    ctx.profile(True)
+    for _ in range(iters + 2):
+        reset()
+        ctx.synchronize()
+        t = rf.Timer(ctx)
+        t.begin()
+        ctx.sort_pairs(work["rk"], work["rv"], n_r, 4, 32, 4)
+        ctx.sort_pairs(work["sk"], work["sv"], n_s, 4, 32, 4)
+        mid = rf.Timer(ctx)
+        mid.begin()
+        ctx.join_find_bounds(lb, lc, tot, work["rk"], n_r, work["sk"], n_s, 4)
+        ctx.join_expand_gather(out, 16, work["rk"], 4, work["rv"], 4, work["sv"], 4, lb, lc,
+                               n_r, tot, total, 64, 32)
+        core.append(mid.end())
+        plan.append(t.end())
+        t.close()
+        mid.close()
+    fb = ctx.profile_read("join.find_bounds")
+    eg = ctx.profile_read("join.expand_gather")
+    ctx.profile(False)
+    core_ms, plan_ms = median(core[2:]), median(plan[2:])
+    core_bytes = 8 * n_r + 4 * n_s + 16 * total  # keys+lb/count, right keys, output
+    res = {"R": n_r, "S": n_s, "out_rows": total, "core_ms": core_ms, "plan_ms": plan_ms,
+           "core_rows_per_s": (n_r + n_s) / (core_ms * 1e-3),
+           "plan_rows_per_s": (n_r + n_s) / (plan_ms * 1e-3),
+           "core_GBps": core_bytes / (core_ms * 1e-3) / 1e9,
+           "find_bounds_ms": fb[1] / max(fb[0], 1), "expand_gather_ms": eg[1] / max(eg[0], 1)}
+    # expand+gather algorithmic bytes: lower bound + count per left row,
+    # r_val per left row, s_val per output (gathered), 16 B out per output.
+    eg_bytes = 12 * n_r + 4 * total + 16 * total
+    res["expand_gather_roofline"] = roofline(eg_bytes, eg[0], eg[1])
+    for b in list(src.values()) + list(work.values()) + [lb, lc, tot, out]:
+        b.close()
+    return res
+
+
+def bench_q1(rf, ctx, iters=5):
+    n = 60_000_000
+    cols = [rf.DeviceBuffer(4 * n), rf.DeviceBuffer(n), rf.DeviceBuffer(n), rf.DeviceBuffer(4 * n),
+            rf.DeviceBuffer(8 * n), rf.DeviceBuffer(8 * n), rf.DeviceBuffer(8 * n)]
+    ctx.gen_lineitem(cols, n, SEED + 2)
+    out = rf.DeviceBuffer(48 * 8)
+    ctx.q1(out, *cols, n, 2436)
+    ctx.synchronize()
+    ctx.profile(True)
+    times = []
+    for _ in range(iters):
+        t = rf.Timer(ctx)
+        t.begin()
+        ctx.q1(out, *cols, n, 2436)
+        times.append(t.end())
+        t.close()
+    launches, ms = ctx.profile_read("q1.partial")
+    ctx.profile(False)
+    res = out.download(np.float64, 48).reshape(6, 8)
+    for b in cols + [out]:
+        b.close()
+    m = median(times)
+    return {"n": n, "ms": m, "rows_per_s": n / (m * 1e-3),
+            "roofline": roofline(34 * n, launches, ms),
+            "kept_rows": int(res[:, 5].sum())}
+
+
+def cpu_baseline_sort():
+    """The oracle's serial LSD sort (one thread) on 2^26 of the 2^28 pairs."""
+    import oracle
+    from redfox_amd import datagen
+
+    n = 1 << 26
+    keys = datagen.u32(SEED + 1, n)
+    vals = np.arange(n, dtype=np.uint32)
+    t0 = time.perf_counter()
+    oracle.sort_pairs(keys, vals, 4, 32, 4)
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt, 1), "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"2^26 (u32 key, u32 value) pairs of the same generator, serial LSD radix "
+                      f"sort in oracle/rfx_oracle.c, {dt:.2f} s on "
+                      f"{os.cpu_count()} visible host cores (1 used)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-ops", action="store_true", help="headline sort only")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not supported "
+              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+    import redfox_amd as rf
+
+    if rf.device_count() < 1:
+        raise SystemExit("bench.py: no HIP device visible")
+    ctx = rf.Context(0)
+    log("sort 2^28 u32 pairs ...")
+    s = bench_sort_u32(rf, ctx, args.steps, args.warmup)
+    log(f"  {s['ms_per_step']:.3f} ms/step, {s['rows_per_s'] / 1e9:.2f} G rows/s, "
+        f"onesweep {s['roofline']['launch_ms']:.3f} ms/launch frac {s['roofline']['frac']}")
+    ops = {"sort_u32_2^28": s}
+    if not args.no_ops:
+        log("sort 2^27 u64 Zipf(1.1) pairs ...")
+        ops["sort_u64_zipf_2^27"] = bench_sort_u64_zipf(rf, ctx)
+        log("join config 1 ...")
+        ops["join_65536x262144"] = bench_join(rf, ctx, 65536, 262144, 16, 20)
+        log("join 2^24 x 2^26 ...")
+        ops["join_2^24x2^26"] = bench_join(rf, ctx, 1 << 24, 1 << 26, 24, 5)
+        log("q1 60M ...")
+        ops["q1_60M"] = bench_q1(rf, ctx)
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline_sort()
+    line = {
+        "metric": METRIC,
+        "value": round(s["rows_per_s"], 1),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (splitmix64, device-generated, resident in HBM)",
+        "config": {"workload": "stable LSD radix sort of 2^28 (u32 key uniform over the full "
+                               "range, u32 value = row index) pairs",
+                   "rows": 1 << 28, "key_bits": 32, "value_bytes": 4},
+        "roofline": s["roofline"],
+        "cpu_baseline": cpu,
+        "ops": ops,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

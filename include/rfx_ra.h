@@ -93,6 +93,15 @@ int rfx_memset(void* dst, int value, uint64_t bytes);
 int rfx_device_synchronize(void);
 int rfx_mem_info(uint64_t* free_bytes, uint64_t* total_bytes);
 
+/* Per-launch timing: while enabled, every kernel launch of the operators
+ * is bracketed by HIP events on the context's stream, grouped by tag
+ * ("sort.onesweep", "sort.hist", "join.find_bounds", "join.expand",
+ * "rbk.up", "rbk.down", "setop", "select", "q1.partial", ...).  Enabling
+ * (or disabling) resets the totals.  Reading waits for pending launches. */
+int rfx_ctx_profile(rfx_ctx* ctx, int enable);
+int rfx_ctx_profile_read(rfx_ctx* ctx, const char* tag, uint64_t* launches,
+                         double* total_ms);
+
 /* Events on the context's stream; elapsed time in milliseconds. */
 int rfx_event_create(void** ev);
 int rfx_event_destroy(void* ev);
@@ -106,6 +115,19 @@ int rfx_gen_u32(rfx_ctx* ctx, uint32_t* out, uint64_t n, uint64_t seed);
 int rfx_gen_u64(rfx_ctx* ctx, uint64_t* out, uint64_t n, uint64_t seed);
 /* out[i] = i (u32 row index). */
 int rfx_gen_iota_u32(rfx_ctx* ctx, uint32_t* out, uint64_t n);
+/* out[i] = mix64(rank), rank ~ Zipf(s) over [1, n_max] by rejection-
+ * inversion (Hormann & Derflinger 1996), element i drawing from its own
+ * splitmix64 sub-stream; mix64 is the splitmix64 finaliser (a bijection),
+ * so all 64 bits vary and equal ranks give equal keys. */
+int rfx_gen_zipf_u64(rfx_ctx* ctx, uint64_t* out, uint64_t n, uint64_t seed,
+                     double s, uint64_t n_max);
+/* TPC-H lineitem columns for Q1 from seed (the same values as the host
+ * generator in redfox_amd/datagen.py): shipdate U[0,2526), returnflag
+ * U{0,1,2}, linestatus U{0,1}, qty U[1,50], price = qty * U{90000..200000}
+ * / 100, disc U{0..10} / 100, tax U{0..8} / 100. */
+int rfx_gen_lineitem(rfx_ctx* ctx, int32_t* shipdate, uint8_t* returnflag,
+                     uint8_t* linestatus, int32_t* qty, double* price,
+                     double* disc, double* tax, uint64_t n, uint64_t seed);
 
 /* ---------------------------------------------------------------------- */
 /* Native operators                                                       */

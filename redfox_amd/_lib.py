@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_char_p, c_float, c_int, c_uint32, c_uint64, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_uint32, c_uint64, c_void_p
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librfx_hip.so")
@@ -42,6 +42,8 @@ SIGNATURES = {
     "rfx_memset": (I, [P, I, U64]),
     "rfx_device_synchronize": (I, []),
     "rfx_mem_info": (I, [U64P, U64P]),
+    "rfx_ctx_profile": (I, [P, I]),
+    "rfx_ctx_profile_read": (I, [P, c_char_p, U64P, POINTER(c_double)]),
     "rfx_event_create": (I, [POINTER(c_void_p)]),
     "rfx_event_destroy": (I, [P]),
     "rfx_event_record": (I, [P, P]),
@@ -49,6 +51,8 @@ SIGNATURES = {
     "rfx_gen_u32": (I, [P, P, U64, U64]),
     "rfx_gen_u64": (I, [P, P, U64, U64]),
     "rfx_gen_iota_u32": (I, [P, P, U64]),
+    "rfx_gen_zipf_u64": (I, [P, P, U64, U64, c_double, U64]),
+    "rfx_gen_lineitem": (I, [P, P, P, P, P, P, P, P, U64, U64]),
     "rfx_sort_pairs": (I, [P, P, P, U64, I, I, I]),
     "rfx_sort_workspace": (U64, [U64, I, I, I]),
     "rfx_join_find_bounds": (I, [P, P, P, P, P, U64, P, U64, I]),

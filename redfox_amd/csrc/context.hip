@@ -1,5 +1,6 @@
 // context.hip - contexts, workspace, memory and timing plumbing, error
 // reporting, and the device-side synthetic data generators.
+#include <cmath>
 #include <cstdarg>
 #include <cstdlib>
 #include <mutex>
@@ -43,6 +44,44 @@ int ensure_workspace(rfx_ctx* ctx, uint64_t bytes) {
   }
   ctx->ws = p;
   ctx->ws_bytes = want;
+  return RFX_OK;
+}
+
+static hipEvent_t prof_event(rfx_ctx* ctx) {
+  if (!ctx->prof_pool.empty()) {
+    hipEvent_t e = ctx->prof_pool.back();
+    ctx->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void prof_begin(rfx_ctx* ctx, const char* tag) {
+  rfx_prof_rec r{tag, prof_event(ctx), prof_event(ctx)};
+  (void)hipEventRecord(r.start, ctx->stream);
+  ctx->prof_pending.push_back(r);
+}
+
+void prof_end(rfx_ctx* ctx) {
+  if (ctx->prof_pending.empty()) return;
+  (void)hipEventRecord(ctx->prof_pending.back().stop, ctx->stream);
+}
+
+// Fold finished launches into the per-tag totals.
+static int prof_collect(rfx_ctx* ctx) {
+  for (auto& r : ctx->prof_pending) {
+    float ms = 0.f;
+    RFX_HIP(hipEventSynchronize(r.stop));
+    RFX_HIP(hipEventElapsedTime(&ms, r.start, r.stop));
+    auto& st = ctx->prof_stats[r.tag];
+    st.first += 1;
+    st.second += ms;
+    ctx->prof_pool.push_back(r.start);
+    ctx->prof_pool.push_back(r.stop);
+  }
+  ctx->prof_pending.clear();
   return RFX_OK;
 }
 
@@ -96,6 +135,61 @@ __global__ void k_ones_u32(uint32_t* out, uint64_t n) {
     out[i] = 1u;
 }
 
+// Zipf(s) over [1, n_max] by rejection-inversion (Hormann & Derflinger,
+// "Rejection-inversion to generate variates from monotone discrete
+// distributions", 1996).  Element i tries draws i*64 + 0, 1, ... of its
+// stream until one is accepted.
+struct ZipfParams {
+  double q, s_const, h_x1, h_n;
+  uint64_t n_max;
+};
+__device__ __forceinline__ double zipf_H(double x, double q) { return (exp(q * log(x)) - 1.0) / q; }
+__device__ __forceinline__ double zipf_Hinv(double y, double q) { return exp(log1p(q * y) / q); }
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void k_gen_zipf(uint64_t* out, uint64_t n, uint64_t seed, ZipfParams zp) {
+  uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const double s = 1.0 - zp.q;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += stride) {
+    double k = 1.0;
+    for (int a = 0; a < 64; ++a) {
+      const double u = static_cast<double>(splitmix64(seed, i * 64 + a) >> 11) * (1.0 / 9007199254740992.0);
+      const double ur = zp.h_n + u * (zp.h_x1 - zp.h_n);
+      const double x = zipf_Hinv(ur, zp.q);
+      k = floor(x + 0.5);
+      if (k < 1.0) k = 1.0;
+      if (k > static_cast<double>(zp.n_max)) k = static_cast<double>(zp.n_max);
+      if (k - x <= zp.s_const || ur >= zipf_H(k + 0.5, zp.q) - exp(-s * log(k))) break;
+    }
+    out[i] = mix64(static_cast<uint64_t>(k));
+  }
+}
+
+__global__ void k_gen_lineitem(int32_t* shipdate, uint8_t* flag, uint8_t* status, int32_t* qty,
+                               double* price, double* disc, double* tax, uint64_t n,
+                               uint64_t seed) {
+  uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += stride) {
+    const uint64_t a = splitmix64(seed, i), b = splitmix64(seed + 1, i);
+    shipdate[i] = static_cast<int32_t>((a >> 32) % 2526);
+    flag[i] = static_cast<uint8_t>(((a >> 16) & 0xFFFF) % 3);
+    status[i] = static_cast<uint8_t>(((a >> 8) & 0xFF) % 2);
+    const int32_t q = static_cast<int32_t>((a & 0xFF) % 50 + 1);
+    qty[i] = q;
+    const uint64_t cents = ((b >> 32) % 110001) + 90000;
+    price[i] = static_cast<double>(q) * (static_cast<double>(cents) / 100.0);
+    disc[i] = static_cast<double>(((b >> 16) & 0xFFFF) % 11) / 100.0;
+    tax[i] = static_cast<double>((b & 0xFFFF) % 9) / 100.0;
+  }
+}
+
 static uint32_t grid_for(uint64_t n) {
   uint64_t b = (n + 255) / 256;
   if (b > 8192) b = 8192;
@@ -143,6 +237,8 @@ int rfx_ctx_create(rfx_ctx** out, int device, void* stream) {
 
 int rfx_ctx_destroy(rfx_ctx* ctx) {
   if (!ctx) return RFX_OK;
+  (void)prof_collect(ctx);
+  for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
   if (ctx->ws) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(ctx->ws);
@@ -169,6 +265,23 @@ uint64_t rfx_ctx_workspace_bytes(rfx_ctx* ctx) { return ctx ? ctx->ws_bytes : 0;
 int rfx_ctx_synchronize(rfx_ctx* ctx) {
   if (!ctx) return fail(RFX_EINVAL, "null context");
   RFX_HIP(hipStreamSynchronize(ctx->stream));
+  return RFX_OK;
+}
+
+int rfx_ctx_profile(rfx_ctx* ctx, int enable) {
+  if (!ctx) return fail(RFX_EINVAL, "null context");
+  RFX_TRY(prof_collect(ctx));
+  ctx->prof_stats.clear();
+  ctx->profiling = enable != 0;
+  return RFX_OK;
+}
+
+int rfx_ctx_profile_read(rfx_ctx* ctx, const char* tag, uint64_t* launches, double* total_ms) {
+  if (!ctx || !tag) return fail(RFX_EINVAL, "null argument");
+  RFX_TRY(prof_collect(ctx));
+  auto it = ctx->prof_stats.find(tag);
+  if (launches) *launches = it == ctx->prof_stats.end() ? 0 : it->second.first;
+  if (total_ms) *total_ms = it == ctx->prof_stats.end() ? 0.0 : it->second.second;
   return RFX_OK;
 }
 
@@ -277,6 +390,36 @@ int rfx_gen_iota_u32(rfx_ctx* ctx, uint32_t* out, uint64_t n) {
   RFX_LAUNCHED();
   return RFX_OK;
 }
+int rfx_gen_zipf_u64(rfx_ctx* ctx, uint64_t* out, uint64_t n, uint64_t seed, double s,
+                     uint64_t n_max) {
+  if (!ctx || (!out && n)) return fail(RFX_EINVAL, "rfx_gen_zipf_u64: null argument");
+  if (!(s > 1.0) || n_max < 1) return fail(RFX_EINVAL, "rfx_gen_zipf_u64: s must exceed 1");
+  if (!n) return RFX_OK;
+  ZipfParams zp;
+  zp.q = 1.0 - s;
+  auto H = [&](double x) { return (std::exp(zp.q * std::log(x)) - 1.0) / zp.q; };
+  auto Hinv = [&](double y) { return std::exp(std::log1p(zp.q * y) / zp.q); };
+  zp.h_x1 = H(1.5) - 1.0;
+  zp.h_n = H(static_cast<double>(n_max) + 0.5);
+  zp.s_const = 2.0 - Hinv(H(2.5) - std::exp(-s * std::log(2.0)));
+  zp.n_max = n_max;
+  k_gen_zipf<<<grid_for(n), 256, 0, ctx->stream>>>(out, n, seed, zp);
+  RFX_LAUNCHED();
+  return RFX_OK;
+}
+
+int rfx_gen_lineitem(rfx_ctx* ctx, int32_t* shipdate, uint8_t* returnflag, uint8_t* linestatus,
+                     int32_t* qty, double* price, double* disc, double* tax, uint64_t n,
+                     uint64_t seed) {
+  if (!ctx || (n && (!shipdate || !returnflag || !linestatus || !qty || !price || !disc || !tax)))
+    return fail(RFX_EINVAL, "rfx_gen_lineitem: null argument");
+  if (!n) return RFX_OK;
+  k_gen_lineitem<<<grid_for(n), 256, 0, ctx->stream>>>(shipdate, returnflag, linestatus, qty,
+                                                        price, disc, tax, n, seed);
+  RFX_LAUNCHED();
+  return RFX_OK;
+}
+
 int rfx_generate_ones(rfx_ctx* ctx, uint32_t* out, uint64_t n) {
   if (!ctx || (!out && n)) return fail(RFX_EINVAL, "rfx_generate_ones: null argument");
   if (!n) return RFX_OK;

@@ -350,10 +350,13 @@ int find_bounds_impl(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, co
   k_fb_partition<K><<<static_cast<uint32_t>((ntiles + 1 + 255) / 256), 256, 0, ctx->stream>>>(
       static_cast<const K*>(left), nl, static_cast<const K*>(right), nr, ntiles, split);
   RFX_LAUNCHED();
-  k_find_bounds<K><<<static_cast<uint32_t>(ntiles), kJoinNT, 0, ctx->stream>>>(
-      static_cast<const K*>(left), nl, static_cast<const K*>(right), nr, split,
-      static_cast<uint32_t>(ntiles), lb, lc, total, status, ticket);
-  RFX_LAUNCHED();
+  {
+    RFX_PROF(ctx, "join.find_bounds");
+    k_find_bounds<K><<<static_cast<uint32_t>(ntiles), kJoinNT, 0, ctx->stream>>>(
+        static_cast<const K*>(left), nl, static_cast<const K*>(right), nr, split,
+        static_cast<uint32_t>(ntiles), lb, lc, total, status, ticket);
+    RFX_LAUNCHED();
+  }
   return RFX_OK;
 }
 
@@ -370,6 +373,7 @@ int expand_impl(rfx_ctx* ctx, int mode, int32_t* li, int32_t* ri, const GatherSp
   k_expand_partition<<<static_cast<uint32_t>((ntiles + 1 + 255) / 256), 256, 0, ctx->stream>>>(
       lc, nl, total, total_cap, ntiles, split);
   RFX_LAUNCHED();
+  RFX_PROF(ctx, mode == 0 ? "join.expand" : "join.expand_gather");
   if (mode == 0)
     k_expand<0><<<static_cast<uint32_t>(ntiles), kJoinNT, 0, ctx->stream>>>(
         lb, lc, nl, total, total_cap, split, li, ri, gs);
@@ -444,6 +448,7 @@ extern "C" int rfx_join_gather(rfx_ctx* ctx, void* out, int out_bytes, const voi
   uint64_t blocks = (total_cap + 255) / 256;
   const uint64_t cap = static_cast<uint64_t>(ctx->device_cus) * 16;
   if (blocks > cap) blocks = cap;
+  RFX_PROF(ctx, "join.gather");
   k_gather<<<static_cast<uint32_t>(blocks), 256, 0, ctx->stream>>>(gs, left_idx, right_idx, total,
                                                                    total_cap);
   RFX_LAUNCHED();

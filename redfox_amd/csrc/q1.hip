@@ -153,9 +153,12 @@ extern "C" int rfx_q1(rfx_ctx* ctx, double* out, const int32_t* shipdate,
   RFX_TRY(ensure_workspace(ctx, c0.off));
   Carve w(ctx->ws);
   double* part = w.take<double>(static_cast<uint64_t>(kQGroups) * (kQSums + 1) * kQBlocks);
-  k_q1_partial<<<kQBlocks, kQNT, 0, ctx->stream>>>(shipdate, returnflag, linestatus, qty, price,
-                                                     disc, tax, n, cutoff, part);
-  RFX_LAUNCHED();
+  {
+    RFX_PROF(ctx, "q1.partial");
+    k_q1_partial<<<kQBlocks, kQNT, 0, ctx->stream>>>(shipdate, returnflag, linestatus, qty, price,
+                                                       disc, tax, n, cutoff, part);
+    RFX_LAUNCHED();
+  }
   k_q1_final<<<kQGroups * (kQSums + 1), 1024, 0, ctx->stream>>>(part, out);
   RFX_LAUNCHED();
   k_q1_avgs<<<1, 64, 0, ctx->stream>>>(out);

@@ -296,11 +296,15 @@ int rbk_impl(rfx_ctx* ctx, void* out_keys, void* out_vals, const void* keys, con
   zz.n_saved = w.take<uint64_t>(1);
   const uint64_t tiles_cap = (z.n_cap + kRTile - 1) / kRTile;
   const uint32_t grid = static_cast<uint32_t>(tiles_cap < kRChunks ? tiles_cap : kRChunks);
-  k_rbk_up<K, V, Op, ONES><<<grid, kRNT, 0, ctx->stream>>>(
-      static_cast<const K*>(keys), static_cast<const V*>(vals), zz, agg);
-  RFX_LAUNCHED();
+  {
+    RFX_PROF(ctx, "rbk.up");
+    k_rbk_up<K, V, Op, ONES><<<grid, kRNT, 0, ctx->stream>>>(
+        static_cast<const K*>(keys), static_cast<const V*>(vals), zz, agg);
+    RFX_LAUNCHED();
+  }
   k_rbk_spine<V, Op><<<1, kRChunks, 0, ctx->stream>>>(agg, zz);
   RFX_LAUNCHED();
+  RFX_PROF(ctx, "rbk.down");
   k_rbk_down<K, V, Op, ONES><<<grid, kRNT, 0, ctx->stream>>>(
       static_cast<const K*>(keys), static_cast<const V*>(vals), zz, agg, static_cast<K*>(out_keys),
       static_cast<V*>(out_vals));
@@ -328,6 +332,7 @@ int single_impl(rfx_ctx* ctx, V* result, const V* v, uint64_t n) {
   RFX_TRY(ensure_workspace(ctx, w0.off));
   Carve w(ctx->ws);
   V* part = w.take<V>(kRChunks);
+  RFX_PROF(ctx, "reduce.single");
   k_reduce_up<V, Op><<<kRChunks, 256, 0, ctx->stream>>>(v, n, part);
   RFX_LAUNCHED();
   k_reduce_final<V, Op><<<1, kRChunks, 0, ctx->stream>>>(part, result);

@@ -5,9 +5,16 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <vector>
 
 #include "../../include/rfx_ra.h"
+
+struct rfx_prof_rec {
+  const char* tag;
+  hipEvent_t start, stop;
+};
 
 struct rfx_ctx {
   int device = 0;
@@ -15,6 +22,11 @@ struct rfx_ctx {
   void* ws = nullptr;         // cached device workspace
   uint64_t ws_bytes = 0;
   int device_cus = 256;       // compute units, for persistent grids
+  // Per-launch timing (off unless rfx_ctx_profile enabled it).
+  bool profiling = false;
+  std::vector<rfx_prof_rec> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
+  std::map<std::string, std::pair<uint64_t, double>> prof_stats;  // launches, ms
 };
 
 namespace rfx {
@@ -100,6 +112,20 @@ int reduce_by_key_sized(rfx_ctx* ctx, int op, int val_kind, int kb, void* out_ke
                         uint64_t out_a_mul, uint64_t* out_b, uint64_t out_b_mul);
 // kind 0: f64 sum, 1: u32 sum, 2: max of u64 bit patterns.
 int single_reduce(rfx_ctx* ctx, int kind, void* result, const void* v, uint64_t n);
+
+// Bracket a kernel launch with events when the context is profiling.
+void prof_begin(rfx_ctx* ctx, const char* tag);
+void prof_end(rfx_ctx* ctx);
+struct ProfScope {
+  rfx_ctx* c;
+  ProfScope(rfx_ctx* ctx, const char* tag) : c(ctx) {
+    if (c->profiling) prof_begin(c, tag);
+  }
+  ~ProfScope() {
+    if (c->profiling) prof_end(c);
+  }
+};
+#define RFX_PROF(ctx, tag) ::rfx::ProfScope rfx_prof_scope_##__LINE__((ctx), (tag))
 
 // Abort with the message, as the reference's check() does.
 [[noreturn]] void die(const char* where, int status);

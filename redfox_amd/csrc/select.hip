@@ -196,6 +196,7 @@ int select_impl(rfx_ctx* ctx, void* out, uint64_t* n_out, const void* in, uint64
   uint32_t* ticket = w.take<uint32_t>(1);
   RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(status),
                    (reinterpret_cast<char*>(ticket + 1) - reinterpret_cast<char*>(status)) / 4, 0));
+  RFX_PROF(ctx, "select");
   k_select<K><<<static_cast<uint32_t>(ntiles), kFNT, 0, ctx->stream>>>(
       static_cast<const K*>(in), n, static_cast<uint32_t>(ntiles), p, static_cast<K*>(out), n_out,
       status, ticket);

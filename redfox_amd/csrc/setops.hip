@@ -284,6 +284,7 @@ int set_op_impl(rfx_ctx* ctx, int op, void* out, uint64_t* n_out, uint64_t mul, 
   RFX_LAUNCHED();
   const uint32_t g = static_cast<uint32_t>(ntiles);
   K* O = static_cast<K*>(out);
+  RFX_PROF(ctx, "setop");
   switch (op) {
     case RFX_SET_UNION:
       k_set_op<K, RFX_SET_UNION><<<g, kSNT, 0, ctx->stream>>>(A, na, B, nb, w.split, g, O, n_out,
@@ -312,6 +313,7 @@ int unique_impl(rfx_ctx* ctx, void* out, uint64_t* n_out, uint64_t mul, const vo
   void* tmp = nullptr;
   RFX_TRY(lb_workspace(ctx, ntiles, in_place ? n * sizeof(K) : 0, &w, &tmp));
   K* dst = static_cast<K*>(in_place ? tmp : out);
+  RFX_PROF(ctx, "unique");
   k_unique<K><<<static_cast<uint32_t>(ntiles), kSNT, 0, ctx->stream>>>(
       static_cast<const K*>(in), n, static_cast<uint32_t>(ntiles), dst, n_out, mul, w.status,
       w.ticket);

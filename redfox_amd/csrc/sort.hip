@@ -324,16 +324,20 @@ int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
   uint64_t hmax = static_cast<uint64_t>(ctx->device_cus) * 8;
   if (hb > hmax) hb = hmax;
   if (hb == 0) hb = 1;
-  if (reinterpret_cast<uintptr_t>(keys) % 16 == 0)
-    k_radix_hist<KB, true><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
-        static_cast<const K*>(keys), n, passes, static_cast<uint32_t>(key_bits), L.hist);
-  else
-    k_radix_hist<KB, false><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
-        static_cast<const K*>(keys), n, passes, static_cast<uint32_t>(key_bits), L.hist);
-  RFX_LAUNCHED();
+  {
+    RFX_PROF(ctx, "sort.hist");
+    if (reinterpret_cast<uintptr_t>(keys) % 16 == 0)
+      k_radix_hist<KB, true><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
+          static_cast<const K*>(keys), n, passes, static_cast<uint32_t>(key_bits), L.hist);
+    else
+      k_radix_hist<KB, false><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
+          static_cast<const K*>(keys), n, passes, static_cast<uint32_t>(key_bits), L.hist);
+    RFX_LAUNCHED();
+  }
   k_radix_scan<<<passes, 256, 0, ctx->stream>>>(L.hist, L.base, L.skip, n);
   RFX_LAUNCHED();
   for (int p = 0; p < passes; ++p) {
+    RFX_PROF(ctx, "sort.onesweep");
     k_onesweep<KB, VB, Tn::NT, Tn::IPT><<<static_cast<uint32_t>(tiles), Tn::NT, 0, ctx->stream>>>(
         static_cast<K*>(keys), static_cast<K*>(L.pong_keys), static_cast<V*>(vals),
         static_cast<V*>(L.pong_vals), n, p, static_cast<uint32_t>(key_bits), L.base, L.ctrs,

@@ -9,7 +9,7 @@ the reference would abort.
 from __future__ import annotations
 
 import ctypes
-from ctypes import byref, c_float, c_int, c_uint64, c_void_p
+from ctypes import byref, c_double, c_float, c_int, c_uint64, c_void_p
 from typing import Optional
 
 import numpy as np
@@ -142,6 +142,25 @@ class Context:
 
     def iota_u32(self, buf: DeviceBuffer, n: int) -> None:
         _check("rfx_gen_iota_u32", lib().rfx_gen_iota_u32(self.handle, buf.ptr, n))
+
+    def gen_zipf_u64(self, buf: DeviceBuffer, n: int, seed: int, s: float, n_max: int) -> None:
+        _check("rfx_gen_zipf_u64", lib().rfx_gen_zipf_u64(self.handle, buf.ptr, n, seed, s, n_max))
+
+    def gen_lineitem(self, cols, n: int, seed: int) -> None:
+        _check("rfx_gen_lineitem", lib().rfx_gen_lineitem(
+            self.handle, *[c.ptr for c in cols], n, seed))
+
+    # -- per-launch timing -------------------------------------------------
+    def profile(self, enable: bool) -> None:
+        _check("rfx_ctx_profile", lib().rfx_ctx_profile(self.handle, int(enable)))
+
+    def profile_read(self, tag: str):
+        """(launches, total milliseconds) recorded for `tag` since profile()."""
+        n = c_uint64(0)
+        ms = c_double(0.0)
+        _check("rfx_ctx_profile_read",
+               lib().rfx_ctx_profile_read(self.handle, tag.encode(), byref(n), byref(ms)))
+        return int(n.value), float(ms.value)
 
     # -- sort (b40cSort.cu / ModernGPUSort.cu) ------------------------------
     def sort_pairs(self, keys: DeviceBuffer, vals: Optional[DeviceBuffer], n: int,
