@@ -182,6 +182,38 @@ __device__ __forceinline__ uint32_t lookback(const uint32_t* status, uint32_t ti
   return excl;
 }
 
+// The same walk, polling W predecessors per round trip: tiles k, k-1, ...,
+// k-W+1 are loaded together and consumed in order up to the first word not
+// yet published (poll again from there) or the first inclusive prefix
+// (done).  Tiles before tile 0 count as an inclusive prefix of 0.
+template <int W>
+__device__ __forceinline__ uint32_t lookback_window(const uint32_t* status, uint32_t tile,
+                                                    uint32_t stride, uint32_t col) {
+  uint32_t excl = 0;
+  int64_t k = static_cast<int64_t>(tile) - 1;
+  while (true) {
+    uint32_t v[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i)
+      v[i] = (k - i >= 0) ? status_load(status + static_cast<uint64_t>(k - i) * stride + col)
+                          : kFlagIncl;
+    bool stop = false, fin = false;
+    int used = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      const uint32_t f = v[i] & ~kCountMask;
+      const bool take = !stop && f != 0;
+      excl += take ? (v[i] & kCountMask) : 0u;
+      used += take;
+      fin |= take && f == kFlagIncl;
+      stop |= (f == 0) || (take && f == kFlagIncl);
+    }
+    if (fin) return excl;
+    k -= used;
+    if (used < W) __builtin_amdgcn_s_sleep(1);
+  }
+}
+
 // 64-bit variant for prefixes that may exceed 2^30 (compaction offsets):
 // status is a u64 word, flag in bits 62-63, value in the low 62 bits.
 constexpr uint64_t kFlagAgg64 = 1ull << 62;

@@ -29,13 +29,23 @@ constexpr int kBins = 256;
 #ifndef RFX_SORT_IPT
 #define RFX_SORT_IPT 16
 #endif
+#ifndef RFX_SORT_MINW
+#define RFX_SORT_MINW 1
+#endif
+#ifndef RFX_SORT_LBW
+#define RFX_SORT_LBW 8
+#endif
 // Tile shape: NT threads x IPT keys per thread; 16-byte elements halve IPT
 // to stay within the LDS reorder buffer and VGPR budget.
 template <int KB, int VB>
 struct SortTune {
   static constexpr int kMax = KB > VB ? KB : VB;
   static constexpr int NT = RFX_SORT_NT;
-  static constexpr int IPT = kMax >= 16 ? RFX_SORT_IPT / 2 : RFX_SORT_IPT;
+  // LDS per block is about kMax*TILE (reorder) + TILE (digits) + 8 KB
+  // (wave counters): 16 keys per thread for <= 4-byte elements (3 blocks
+  // per CU), 14 for 8-byte (2 blocks), 8 for 16-byte.
+  static constexpr int IPT = kMax >= 16 ? RFX_SORT_IPT / 2
+                             : kMax >= 8 ? (RFX_SORT_IPT * 7) / 8 : RFX_SORT_IPT;
   static constexpr int TILE = NT * IPT;
 };
 
@@ -121,7 +131,7 @@ __global__ __launch_bounds__(256) void k_radix_scan(const uint32_t* __restrict__
 }
 
 template <int KB, int VB, int NT, int IPT>
-__global__ __launch_bounds__(NT) void k_onesweep(
+__global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
     typename Elem<KB>::T* __restrict__ k0, typename Elem<KB>::T* __restrict__ k1,
     typename Elem<VB>::T* __restrict__ v0, typename Elem<VB>::T* __restrict__ v1, uint64_t n,
     int pass, uint32_t key_bits, const uint32_t* __restrict__ base, uint32_t* __restrict__ ctrs,
@@ -146,6 +156,7 @@ __global__ __launch_bounds__(NT) void k_onesweep(
   __shared__ uint32_t gdelta[kBins];
   __shared__ uint32_t scan_tmp[NW + 1];
   __shared__ __attribute__((aligned(16))) unsigned char rbuf[RB];
+  __shared__ uint8_t sdig[VB > 0 ? TILE : 1];  // digit of each reordered row
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
@@ -160,18 +171,13 @@ __global__ __launch_bounds__(NT) void k_onesweep(
   const uint64_t tile_base = static_cast<uint64_t>(tile) * TILE;
   const uint64_t wave_base = tile_base + static_cast<uint64_t>(w) * kWave * IPT;
 
+  // Keys only: values are read after the keys are scattered, which keeps
+  // the ranking phase's register footprint small.
   K keys[IPT];
-  V vals[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
-    if (idx < n) {
-      keys[j] = kin[idx];
-      if constexpr (VB > 0) vals[j] = vin[idx];
-    } else {
-      keys[j] = K{};
-      if constexpr (VB > 0) vals[j] = V{};
-    }
+    const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+    keys[j] = idx < n ? kin[idx] : K{};
   }
 
   // Stable rank within the wave: lanes sharing a digit are found by one
@@ -179,7 +185,7 @@ __global__ __launch_bounds__(NT) void k_onesweep(
   uint32_t pos[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+    const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
     const bool valid = idx < n;
     const uint32_t d = digit_of(keys[j], shift, dmask);
     uint64_t m = ballot(valid);
@@ -202,14 +208,14 @@ __global__ __launch_bounds__(NT) void k_onesweep(
   }
   __syncthreads();
 
-  // Per-digit tile totals, wave prefixes, and the look-back.
+  // Per-digit tile totals and wave prefixes; publish the tile aggregate.
   uint32_t total_d = 0;
   uint32_t* st = status + (static_cast<uint64_t>(pass) * ntiles) * kBins;
   if (tid < kBins) {
     uint32_t run = 0;
 #pragma unroll
     for (int ww = 0; ww < NW; ++ww) {
-      uint32_t c = wcnt[ww][tid];
+      const uint32_t c = wcnt[ww][tid];
       wcnt[ww][tid] = run;
       run += c;
     }
@@ -219,61 +225,75 @@ __global__ __launch_bounds__(NT) void k_onesweep(
   }
   uint32_t all;
   const uint32_t bs = block_excl_scan<NT>(tid < kBins ? total_d : 0u, scan_tmp, &all);
-  if (tid < kBins) {
-    bstart[tid] = bs;
-    uint32_t excl = 0;
-    if (tile > 0) {
-      excl = lookback(st, tile, kBins, tid);
-      status_store(st + static_cast<uint64_t>(tile) * kBins + tid, kFlagIncl | (excl + total_d));
-    }
-    gdelta[tid] = base[pass * kBins + tid] + excl - bs;
-  }
+  if (tid < kBins) bstart[tid] = bs;
   __syncthreads();
 
-  // Reorder through LDS so each digit's run is written contiguously.
+  // Work that does not need the global offsets goes before the look-back so
+  // it overlaps the wait: the LDS reorder of the keys and the value loads.
   K* kbuf = reinterpret_cast<K*>(rbuf);
 #pragma unroll
   for (int j = 0; j < IPT; ++j) {
-    uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+    const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
     if (idx < n) {
       const uint32_t d = digit_of(keys[j], shift, dmask);
       pos[j] += bstart[d] + wcnt[w][d];
       kbuf[pos[j]] = keys[j];
     }
   }
+  V vals[IPT];
+  if constexpr (VB > 0) {
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+      if (idx < n) vals[j] = vin[idx];
+    }
+  }
+  if (tid < kBins) {
+    uint32_t excl = 0;
+    if (tile > 0) {
+      excl = lookback_window<RFX_SORT_LBW>(st, tile, kBins, tid);
+      status_store(st + static_cast<uint64_t>(tile) * kBins + tid, kFlagIncl | (excl + total_d));
+    }
+    gdelta[tid] = base[pass * kBins + tid] + excl - bs;
+  }
   __syncthreads();
+
   const uint32_t tile_n =
       static_cast<uint32_t>((n - tile_base) < TILE ? (n - tile_base) : TILE);
-  uint32_t dst[IPT];
 #pragma unroll
   for (int k = 0; k < IPT; ++k) {
     const uint32_t i = k * NT + tid;
     if (i < tile_n) {
       const K key = kbuf[i];
-      dst[k] = gdelta[digit_of(key, shift, dmask)] + i;
-      if (dst[k] < n) kout[dst[k]] = key;  // always true when ranks are right
+      const uint32_t d = digit_of(key, shift, dmask);
+      if constexpr (VB > 0) sdig[i] = static_cast<uint8_t>(d);
+      const uint32_t dst = gdelta[d] + i;
+      if (dst < n) kout[dst] = key;  // always true when ranks are right
     }
   }
   if constexpr (VB > 0) {
-    __syncthreads();
+    __syncthreads();  // every key is out of kbuf
     V* vbuf = reinterpret_cast<V*>(rbuf);
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
-      uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+      const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
       if (idx < n) vbuf[pos[j]] = vals[j];
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
       const uint32_t i = k * NT + tid;
-      if (i < tile_n && dst[k] < n) vout[dst[k]] = vbuf[i];
+      if (i < tile_n) {
+        const uint32_t dst = gdelta[sdig[i]] + i;
+        if (dst < n) vout[dst] = vbuf[i];
+      }
     }
   }
 
   // The last tile to finish flips the buffer selector.  Every block read
   // *sel before arriving here, so nobody reads the new value this pass.
   if (tid == 0) {
-    uint32_t prev = atomicAdd(&ctrs[pass * 2 + 1], 1u);
+    const uint32_t prev = atomicAdd(&ctrs[pass * 2 + 1], 1u);
     if (prev == ntiles - 1)
       __hip_atomic_store(sel, cur ^ 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
