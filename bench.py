@@ -9,8 +9,9 @@ generated before the timed region).  `roofline` covers the dominant kernel,
 the onesweep digit pass, timed per launch with HIP events on its stream
 during the timed steps.  `cpu_baseline` is the oracle's serial LSD sort on a
 bounded sample, on this host.  `ops` adds the other workloads: the u64
-Zipf(1.1) sort, the config-1 join and an HBM-scale join, and TPC-H Q1 over
-60M rows.
+Zipf(1.1) sort, the config-1 join and an HBM-scale join, TPC-H Q1 over 60M
+rows, and 512 batched join-aggregate instances (plain per-operator launches
+vs one captured graph).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -214,6 +215,44 @@ def bench_q1(rf, ctx, iters=5):
             "kept_rows": int(res[:, 5].sum())}
 
 
+def bench_batched(rf, ctx, reps=5):
+    """512 join + group-aggregate instances (fact 10^U[3,7] rows, Zipf(1.1)
+    foreign keys into dimensions of 10^U[2,5] rows): the plain per-operator
+    form (one instance and one operator launch at a time, sizes read back
+    as Harmony does) against the same plan batched into one HIP graph."""
+    from redfox_amd import batched
+
+    w = batched.Workload(ctx, n_inst=512)
+    plain = batched.PlainRunner(w, ctx)
+    plain.run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    plain.run()
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t0
+    br = batched.BatchedRunner(w)
+    br.capture()
+    times = []
+    for _ in range(reps):
+        br.synchronize()
+        t0 = time.perf_counter()
+        br.run()
+        br.synchronize()
+        times.append(time.perf_counter() - t0)
+    t_b = median(times)
+    a, b = plain.results(), br.results()
+    match = all(np.array_equal(x[0], y[0]) and np.array_equal(x[2], y[2]) and
+                np.allclose(x[1], y[1], rtol=1e-12, atol=0) for x, y in zip(a, b))
+    res = {"instances": w.n_inst, "fact_rows": w.N, "dim_rows": w.D,
+           "plain_s": t_plain, "batched_ms": 1e3 * t_b,
+           "qps_plain": w.n_inst / t_plain, "qps_batched": w.n_inst / t_b,
+           "speedup": t_plain / t_b, "rows_per_s_batched": w.N / t_b,
+           "input_GBps_batched": w.bytes_per_batch() / t_b / 1e9,
+           "results_match": bool(match)}
+    br.graph.close()
+    return res
+
+
 def cpu_baseline_sort():
     """The oracle's serial LSD sort (one thread) on 2^26 of the 2^28 pairs."""
     import oracle
@@ -262,6 +301,8 @@ def main():
         ops["join_2^24x2^26"] = bench_join(rf, ctx, 1 << 24, 1 << 26, 24, 5)
         log("q1 60M ...")
         ops["q1_60M"] = bench_q1(rf, ctx)
+        log("batched 512 join-aggregate instances ...")
+        ops["batched_512"] = bench_batched(rf, ctx)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")

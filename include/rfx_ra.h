@@ -129,6 +129,35 @@ int rfx_gen_lineitem(rfx_ctx* ctx, int32_t* shipdate, uint8_t* returnflag,
                      uint8_t* linestatus, int32_t* qty, double* price,
                      double* disc, double* tax, uint64_t n, uint64_t seed);
 
+/* Streams and graphs: a context bound to its own stream can record a whole
+ * plan (any sequence of native operators) into a HIP graph and replay it
+ * with one launch.  Reserve the workspace before capturing: growing it
+ * during capture fails with RFX_EINVAL.  Profiling must be off. */
+int rfx_stream_create(void** stream);
+int rfx_stream_destroy(void* stream);
+int rfx_stream_synchronize(void* stream);
+int rfx_graph_begin(rfx_ctx* ctx);
+int rfx_graph_end(rfx_ctx* ctx, void** graph_exec);
+int rfx_graph_launch(rfx_ctx* ctx, void* graph_exec);
+int rfx_graph_destroy(void* graph_exec);
+
+/* Device-to-device copy ordered on the context's stream (capturable). */
+int rfx_copy(rfx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Instance-prefixed keys for running many query instances as one:
+ * out[i] = (s << key_bits) | key[i] where s is the segment holding row i
+ * (seg_offsets: nseg + 1 ascending device u64, seg_offsets[0] = 0). */
+int rfx_segment_keys(rfx_ctx* ctx, uint32_t* out, const uint32_t* key,
+                     const uint64_t* seg_offsets, uint32_t nseg, uint64_t n,
+                     uint32_t key_bits);
+/* Fact rows of the batched join-aggregate workload: row i of instance b
+ * (row_off) gets fk = perm[dim_off[b] + rank - 1], rank ~ Zipf(s) over
+ * [1, d_b], d_b = dim_off[b+1] - dim_off[b], and measure =
+ * U{0..100000} / 100, all from splitmix64 stream `seed`. */
+int rfx_gen_fk_batch(rfx_ctx* ctx, uint32_t* fk, double* measure,
+                     const uint64_t* row_off, const uint64_t* dim_off,
+                     const uint32_t* perm, uint32_t nseg, uint64_t n,
+                     uint64_t seed, double s);
+
 /* ---------------------------------------------------------------------- */
 /* Native operators                                                       */
 /* ---------------------------------------------------------------------- */
@@ -180,6 +209,20 @@ int rfx_join_expand_gather(rfx_ctx* ctx, void* out, int out_bytes,
                            const int32_t* left_count, uint64_t nl,
                            const uint64_t* total, uint64_t total_cap,
                            uint32_t key_shift, uint32_t lval_shift);
+
+/* Expansion fused with a projection of the joined row into one or two
+ * columns (the gather + project + split steps of a join feeding an
+ * aggregate, in one pass).  fields holds nfields (<= 8) records of five
+ * u32: output column (0 or 1), source (0 left key, 1 left value, 2 right
+ * value), in_shift, bits, out_shift; out[g] |= ((src >> in_shift) & mask)
+ * << out_shift.  out1 may be NULL. */
+int rfx_join_expand_project(rfx_ctx* ctx, const int32_t* lower_bound,
+                            const int32_t* left_count, uint64_t nl,
+                            const uint64_t* total, uint64_t total_cap,
+                            const void* lkey, int lkey_bytes, const void* lval,
+                            int lval_bytes, const void* rval, int rval_bytes,
+                            void* out0, int out0_bytes, void* out1,
+                            int out1_bytes, int nfields, const uint32_t* fields);
 
 /* Reduce-by-key over runs of equal adjacent keys (reduce_by_key in
  * Reduce.cu:175-1532).  keys/vals hold *n rows (device u64 *n);

@@ -6,12 +6,12 @@ contexts, and mirrors the reference's operator interface for tests and
 benchmarks.  Importing it fails if the library is not built.
 """
 from ._lib import LIB_PATH, RfxError
-from .rfx import Context, DeviceBuffer, Timer, device_count, lib, read_u64, u64_cell
+from .rfx import Context, DeviceBuffer, Graph, Stream, Timer, device_count, lib, read_u64, u64_cell
 
 # Load eagerly so a missing or broken library fails at import time.
 lib()
 
 __all__ = [
-    "LIB_PATH", "RfxError", "Context", "DeviceBuffer", "Timer", "device_count", "lib",
+    "LIB_PATH", "RfxError", "Context", "DeviceBuffer", "Graph", "Stream", "Timer", "device_count", "lib",
     "read_u64", "u64_cell",
 ]

@@ -53,12 +53,23 @@ SIGNATURES = {
     "rfx_gen_iota_u32": (I, [P, P, U64]),
     "rfx_gen_zipf_u64": (I, [P, P, U64, U64, c_double, U64]),
     "rfx_gen_lineitem": (I, [P, P, P, P, P, P, P, P, U64, U64]),
+    "rfx_stream_create": (I, [POINTER(c_void_p)]),
+    "rfx_stream_destroy": (I, [P]),
+    "rfx_stream_synchronize": (I, [P]),
+    "rfx_graph_begin": (I, [P]),
+    "rfx_graph_end": (I, [P, POINTER(c_void_p)]),
+    "rfx_graph_launch": (I, [P, P]),
+    "rfx_graph_destroy": (I, [P]),
+    "rfx_copy": (I, [P, P, P, U64]),
+    "rfx_segment_keys": (I, [P, P, P, P, U32, U64, U32]),
+    "rfx_gen_fk_batch": (I, [P, P, P, P, P, P, U32, U64, U64, c_double]),
     "rfx_sort_pairs": (I, [P, P, P, U64, I, I, I]),
     "rfx_sort_workspace": (U64, [U64, I, I, I]),
     "rfx_join_find_bounds": (I, [P, P, P, P, P, U64, P, U64, I]),
     "rfx_join_main": (I, [P, P, P, P, U64, P, P, U64]),
     "rfx_join_gather": (I, [P, P, I, P, I, P, I, P, I, P, P, P, U64, U32, U32]),
     "rfx_join_expand_gather": (I, [P, P, I, P, I, P, I, P, I, P, P, U64, P, U64, U32, U32]),
+    "rfx_join_expand_project": (I, [P, P, P, U64, P, U64, P, I, P, I, P, I, P, I, P, I, I, P]),
     "rfx_reduce_by_key_f64": (I, [P, P, P, P, P, P, P, U64, I, I]),
     "rfx_reduce_by_key_u32": (I, [P, P, P, P, P, P, P, U64, I]),
     "rfx_count_by_key": (I, [P, P, P, P, P, P, U64, I]),

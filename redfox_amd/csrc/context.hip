@@ -27,6 +27,9 @@ void die(const char* where, int status) {
 
 int ensure_workspace(rfx_ctx* ctx, uint64_t bytes) {
   if (bytes <= ctx->ws_bytes) return RFX_OK;
+  if (ctx->capturing)
+    return fail(RFX_EINVAL, "workspace must grow to %llu bytes during graph capture: reserve first",
+                static_cast<unsigned long long>(bytes));
   RFX_HIP(hipSetDevice(ctx->device));
   if (ctx->ws) {
     // Work already queued may still read the old workspace.

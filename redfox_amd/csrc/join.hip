@@ -238,11 +238,37 @@ __device__ __forceinline__ u128 compose(const GatherSpec& g, uint64_t li, uint64
   return mask128(o, 8u * static_cast<uint32_t>(g.out_bytes));
 }
 
+// Mode 2: the output row is projected straight into up to two columns.
+// Field f takes bits [in_shift, in_shift + bits) of source src (0 left key,
+// 1 left value, 2 right value) to out_shift of output column col.
+struct EmitField {
+  uint8_t col, src, in_shift, bits, out_shift;
+};
+struct EmitSpec {
+  const void* src[3];
+  int src_bytes[3];
+  void* out[2];
+  int out_bytes[2];
+  int nf;
+  EmitField f[8];
+};
+
+__device__ __forceinline__ void emit(const EmitSpec& e, uint64_t g, uint64_t li, uint64_t ri) {
+  u128 o[2] = {{0, 0}, {0, 0}};
+  for (int k = 0; k < e.nf; ++k) {
+    const EmitField f = e.f[k];
+    const u128 v = load_w(e.src[f.src], f.src == 2 ? ri : li, e.src_bytes[f.src]);
+    o[f.col] = or128(o[f.col], shl128(mask128(shr128(v, f.in_shift), f.bits), f.out_shift));
+  }
+  store_w(e.out[0], g, e.out_bytes[0], o[0]);
+  if (e.out[1]) store_w(e.out[1], g, e.out_bytes[1], o[1]);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kJoinNT) void k_expand(
     const int32_t* __restrict__ lower_bound, const int32_t* __restrict__ starts, uint64_t nrows,
     const uint64_t* __restrict__ total_p, uint64_t total_cap, const uint64_t* __restrict__ split,
-    int32_t* __restrict__ left_idx, int32_t* __restrict__ right_idx, GatherSpec gs) {
+    int32_t* __restrict__ left_idx, int32_t* __restrict__ right_idx, GatherSpec gs, EmitSpec es) {
   __shared__ int32_t s_start[kJoinNV + 1];
   __shared__ int32_t s_lb[kJoinNV + 1];
   __shared__ int32_t s_li[kJoinNV];
@@ -303,6 +329,8 @@ __global__ __launch_bounds__(kJoinNT) void k_expand(
     if constexpr (MODE == 0) {
       left_idx[g0 + i] = s_li[i];
       right_idx[g0 + i] = s_ri[i];
+    } else if constexpr (MODE == 2) {
+      emit(es, g0 + i, static_cast<uint32_t>(s_li[i]), static_cast<uint32_t>(s_ri[i]));
     } else {
       store_w(gs.out, g0 + i, gs.out_bytes,
               compose(gs, static_cast<uint32_t>(s_li[i]), static_cast<uint32_t>(s_ri[i])));
@@ -362,7 +390,7 @@ int find_bounds_impl(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, co
 
 int expand_impl(rfx_ctx* ctx, int mode, int32_t* li, int32_t* ri, const GatherSpec& gs,
                 const uint64_t* total, uint64_t total_cap, const int32_t* lb, const int32_t* lc,
-                uint64_t nl) {
+                uint64_t nl, const EmitSpec& es = EmitSpec{}) {
   if (nl == 0 || total_cap == 0) return RFX_OK;
   const uint64_t ntiles = (total_cap + nl + kJoinNV - 1) / kJoinNV;
   Carve c(nullptr);
@@ -373,13 +401,16 @@ int expand_impl(rfx_ctx* ctx, int mode, int32_t* li, int32_t* ri, const GatherSp
   k_expand_partition<<<static_cast<uint32_t>((ntiles + 1 + 255) / 256), 256, 0, ctx->stream>>>(
       lc, nl, total, total_cap, ntiles, split);
   RFX_LAUNCHED();
-  RFX_PROF(ctx, mode == 0 ? "join.expand" : "join.expand_gather");
+  RFX_PROF(ctx, mode == 0 ? "join.expand" : mode == 1 ? "join.expand_gather" : "join.expand_project");
   if (mode == 0)
     k_expand<0><<<static_cast<uint32_t>(ntiles), kJoinNT, 0, ctx->stream>>>(
-        lb, lc, nl, total, total_cap, split, li, ri, gs);
-  else
+        lb, lc, nl, total, total_cap, split, li, ri, gs, es);
+  else if (mode == 1)
     k_expand<1><<<static_cast<uint32_t>(ntiles), kJoinNT, 0, ctx->stream>>>(
-        lb, lc, nl, total, total_cap, split, li, ri, gs);
+        lb, lc, nl, total, total_cap, split, li, ri, gs, es);
+  else
+    k_expand<2><<<static_cast<uint32_t>(ntiles), kJoinNT, 0, ctx->stream>>>(
+        lb, lc, nl, total, total_cap, split, li, ri, gs, es);
   RFX_LAUNCHED();
   return RFX_OK;
 }
@@ -470,4 +501,43 @@ extern "C" int rfx_join_expand_gather(rfx_ctx* ctx, void* out, int out_bytes, co
   if (total_cap + nl >= (1ull << 31))
     return fail(RFX_ERANGE, "rfx_join_expand_gather: output exceeds the int32 index range");
   return expand_impl(ctx, 1, nullptr, nullptr, gs, total, total_cap, lower_bound, left_count, nl);
+}
+
+extern "C" int rfx_join_expand_project(rfx_ctx* ctx, const int32_t* lower_bound,
+                                       const int32_t* left_count, uint64_t nl,
+                                       const uint64_t* total, uint64_t total_cap,
+                                       const void* lkey, int lkey_bytes, const void* lval,
+                                       int lval_bytes, const void* rval, int rval_bytes,
+                                       void* out0, int out0_bytes, void* out1, int out1_bytes,
+                                       int nfields, const uint32_t* fields) {
+  if (!ctx || !total) return fail(RFX_EINVAL, "rfx_join_expand_project: null argument");
+  if (total_cap == 0 || nl == 0) return RFX_OK;
+  if (!lower_bound || !left_count || !out0 || !fields)
+    return fail(RFX_EINVAL, "rfx_join_expand_project: null column");
+  if (nfields < 1 || nfields > 8) return fail(RFX_EINVAL, "rfx_join_expand_project: %d fields", nfields);
+  if (!valid_width(out0_bytes) || (out1 && !valid_width(out1_bytes)))
+    return fail(RFX_EINVAL, "rfx_join_expand_project: output widths");
+  if (total_cap + nl >= (1ull << 31))
+    return fail(RFX_ERANGE, "rfx_join_expand_project: output exceeds the int32 index range");
+  EmitSpec es{};
+  es.src[0] = lkey; es.src[1] = lval; es.src[2] = rval;
+  es.src_bytes[0] = lkey_bytes; es.src_bytes[1] = lval_bytes; es.src_bytes[2] = rval_bytes;
+  es.out[0] = out0; es.out[1] = out1;
+  es.out_bytes[0] = out0_bytes; es.out_bytes[1] = out1 ? out1_bytes : 0;
+  es.nf = nfields;
+  for (int k = 0; k < nfields; ++k) {
+    const uint32_t* f = fields + 5 * k;
+    const uint32_t col = f[0], src = f[1], in_shift = f[2], bits = f[3], out_shift = f[4];
+    if (col > 1 || (col == 1 && !out1) || src > 2 || !es.src[src] || !valid_width(es.src_bytes[src]))
+      return fail(RFX_EINVAL, "rfx_join_expand_project: field %d source/column", k);
+    if (bits == 0 || bits > 128 || in_shift + bits > 8u * es.src_bytes[src] ||
+        out_shift + bits > 8u * es.out_bytes[col])
+      return fail(RFX_EINVAL, "rfx_join_expand_project: field %d bits out of range", k);
+    es.f[k] = EmitField{static_cast<uint8_t>(col), static_cast<uint8_t>(src),
+                        static_cast<uint8_t>(in_shift), static_cast<uint8_t>(bits),
+                        static_cast<uint8_t>(out_shift)};
+  }
+  GatherSpec gs{};
+  return expand_impl(ctx, 2, nullptr, nullptr, gs, total, total_cap, lower_bound, left_count, nl,
+                     es);
 }

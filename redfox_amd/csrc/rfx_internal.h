@@ -24,6 +24,7 @@ struct rfx_ctx {
   int device_cus = 256;       // compute units, for persistent grids
   // Per-launch timing (off unless rfx_ctx_profile enabled it).
   bool profiling = false;
+  bool capturing = false;     // inside rfx_graph_begin/end: no allocation
   std::vector<rfx_prof_rec> prof_pending;
   std::vector<hipEvent_t> prof_pool;
   std::map<std::string, std::pair<uint64_t, double>> prof_stats;  // launches, ms

@@ -150,6 +150,27 @@ class Context:
         _check("rfx_gen_lineitem", lib().rfx_gen_lineitem(
             self.handle, *[c.ptr for c in cols], n, seed))
 
+    # -- streams, graphs, batching -------------------------------------------
+    def copy(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int, dst_off: int = 0,
+             src_off: int = 0) -> None:
+        _check("rfx_copy", lib().rfx_copy(self.handle, dst.ptr + dst_off, src.ptr + src_off, nbytes))
+
+    def segment_keys(self, out, key, seg_offsets, nseg, n, key_bits) -> None:
+        _check("rfx_segment_keys", lib().rfx_segment_keys(
+            self.handle, out.ptr, key.ptr, seg_offsets.ptr, nseg, n, key_bits))
+
+    def gen_fk_batch(self, fk, measure, row_off, dim_off, perm, nseg, n, seed, s) -> None:
+        _check("rfx_gen_fk_batch", lib().rfx_gen_fk_batch(
+            self.handle, fk.ptr, measure.ptr, row_off.ptr, dim_off.ptr, perm.ptr, nseg, n, seed, s))
+
+    def graph_begin(self) -> None:
+        _check("rfx_graph_begin", lib().rfx_graph_begin(self.handle))
+
+    def graph_end(self) -> "Graph":
+        g = c_void_p()
+        _check("rfx_graph_end", lib().rfx_graph_end(self.handle, byref(g)))
+        return Graph(self, g.value)
+
     # -- per-launch timing -------------------------------------------------
     def profile(self, enable: bool) -> None:
         _check("rfx_ctx_profile", lib().rfx_ctx_profile(self.handle, int(enable)))
@@ -199,6 +220,17 @@ class Context:
             lval.ptr if lval is not None else None, lval_bytes,
             rval.ptr if rval is not None else None, rval_bytes, lower_bound.ptr,
             left_count.ptr, nl, total.ptr, total_cap, key_shift, lval_shift))
+
+    def join_expand_project(self, lb, lc, nl, total, total_cap, lkey, lkey_bytes, lval, lval_bytes,
+                            rval, rval_bytes, out0, out0_bytes, out1, out1_bytes, fields) -> None:
+        """fields: list of (column, source, in_shift, bits, out_shift); source 0 = left
+        key, 1 = left value, 2 = right value."""
+        f = np.ascontiguousarray(np.array(fields, dtype=np.uint32).reshape(-1))
+        _check("rfx_join_expand_project", lib().rfx_join_expand_project(
+            self.handle, lb.ptr, lc.ptr, nl, total.ptr, total_cap,
+            lkey.ptr, lkey_bytes, lval.ptr if lval is not None else None, lval_bytes,
+            rval.ptr if rval is not None else None, rval_bytes, out0.ptr, out0_bytes,
+            out1.ptr if out1 is not None else None, out1_bytes, len(fields), f.ctypes.data))
 
     # -- reductions (Reduce.cu) --------------------------------------------
     def reduce_by_key_f64(self, out_keys, out_vals, n_out, keys, vals, n, n_cap,
@@ -269,6 +301,51 @@ class Context:
         _check("rfx_q1", lib().rfx_q1(
             self.handle, out.ptr, shipdate.ptr, flag.ptr, status.ptr, qty.ptr, price.ptr,
             disc.ptr, tax.ptr, n, cutoff))
+
+
+class Stream:
+    """A non-blocking HIP stream (for contexts that capture graphs)."""
+
+    def __init__(self):
+        p = c_void_p()
+        _check("rfx_stream_create", lib().rfx_stream_create(byref(p)))
+        self.handle = p.value
+
+    def synchronize(self) -> None:
+        _check("rfx_stream_synchronize", lib().rfx_stream_synchronize(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().rfx_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """An instantiated HIP graph captured from a context's stream."""
+
+    def __init__(self, ctx: "Context", handle: int):
+        self.ctx = ctx
+        self.handle = handle
+
+    def launch(self) -> None:
+        _check("rfx_graph_launch", lib().rfx_graph_launch(self.ctx.handle, self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().rfx_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Timer:

@@ -166,14 +166,19 @@ def test_gather_variants(rf, ctx, orc, shape):
     lval = datagen.splitmix64(41, 3000 * 2).view(np.uint8)[: 3000 * lvb].view(DT[lvb]).copy() if lvb else None
     rval = datagen.splitmix64(42, 5000 * 2).view(np.uint8)[: 5000 * rvb].view(DT[rvb]).copy() if rvb else None
     want = orc.join_gather(ob, left, kb, lval, lvb, rval, rvb, oli, ori, kshift, lvshift)
-    up = lambda a: rf.DeviceBuffer.from_numpy(a) if a is not None else None  # noqa: E731
+    held = {}  # device buffers stay referenced until the kernels have run
+
+    def up(name, a):
+        held[name] = rf.DeviceBuffer.from_numpy(a) if a is not None else None
+        return held[name]
+
     out = rf.DeviceBuffer(ob * max(ot, 1))
     tot = rf.u64_cell(ot)
-    ctx.join_gather(out, ob, up(left), kb, up(lval), lvb, up(rval), rvb, up(oli), up(ori), tot, ot,
-                    kshift, lvshift)
+    ctx.join_gather(out, ob, up("l", left), kb, up("lv", lval), lvb, up("rv", rval), rvb,
+                    up("li", oli), up("ri", ori), tot, ot, kshift, lvshift)
     fused = rf.DeviceBuffer(ob * max(ot, 1))
-    ctx.join_expand_gather(fused, ob, up(left), kb, up(lval), lvb, up(rval), rvb, up(olb), up(olc),
-                           3000, tot, ot, kshift, lvshift)
+    ctx.join_expand_gather(fused, ob, held["l"], kb, held["lv"], lvb, held["rv"], rvb,
+                           up("lb", olb), up("lc", olc), 3000, tot, ot, kshift, lvshift)
     ctx.synchronize()
     assert np.array_equal(out.download(want.dtype, want.size), want)
     assert np.array_equal(fused.download(want.dtype, want.size), want)
@@ -191,8 +196,32 @@ def test_capacity_clamps_expansion(rf, ctx, orc):
     li.zero()
     ri.zero()
     tot = rf.u64_cell(ot)
-    ctx.join_main(li, ri, tot, cap, rf.DeviceBuffer.from_numpy(olb), rf.DeviceBuffer.from_numpy(olc), 100)
+    d_lb, d_lc = rf.DeviceBuffer.from_numpy(olb), rf.DeviceBuffer.from_numpy(olc)
+    ctx.join_main(li, ri, tot, cap, d_lb, d_lc, 100)
     ctx.synchronize()
     oli, ori = orc.join_main(olb, olc, ot)
     assert np.array_equal(li.download(np.int32, cap), oli[:cap])
     assert not li.download(np.int32, cap + 64)[cap:].any()
+
+
+def test_expand_project_matches_gather_then_project(rf, ctx, orc):
+    """Fused expansion + projection equals gather followed by project."""
+    left = sorted_keys(orc, 4, 20_000, 61, 3000)
+    right = sorted_keys(orc, 4, 8_000, 62, 3000)
+    lval = datagen.splitmix64(63, 20_000)
+    rval = datagen.u32(64, 8_000)
+    olb, olc, ot = orc.join_find_bounds(left, right, 4)
+    oli, ori = orc.join_main(olb, olc, ot)
+    rows = orc.join_gather(16, left, 4, lval, 8, rval, 4, oli, ori, 96, 32)
+    want_k = orc.project(4, rows, 16, [96 + 3, 0], [9, 6], [6, 0])
+    want_v = orc.project(8, rows, 16, [32], [64], [0])
+    up = rf.DeviceBuffer.from_numpy
+    K = rf.DeviceBuffer(4 * ot)
+    V = rf.DeviceBuffer(8 * ot)
+    tot = rf.u64_cell(ot)
+    d_lb, d_lc, d_l, d_lv, d_rv = up(olb), up(olc), up(left), up(lval), up(rval)
+    ctx.join_expand_project(d_lb, d_lc, 20_000, tot, ot, d_l, 4, d_lv, 8, d_rv, 4,
+                            K, 4, V, 8, [(0, 0, 3, 9, 6), (0, 2, 0, 6, 0), (1, 1, 0, 64, 0)])
+    ctx.synchronize()
+    assert np.array_equal(K.download(np.uint32, ot), want_k)
+    assert np.array_equal(V.download(np.uint64, ot), want_v)
