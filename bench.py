@@ -253,6 +253,67 @@ def bench_batched(rf, ctx, reps=5):
     return res
 
 
+def bench_served(rf, ctx, n_req=200, slots=8, rates=(0.25, 0.5, 0.75, 0.9)):
+    """Mixed requests (30% select-project, 30% join-aggregate, 20% three-way
+    join chain, 20% u64 set operations) over device-resident relations of
+    floor(10^U[5,8]) rows: saturation throughput of the continuous server
+    (all requests arriving at once) against the plain one-request-at-a-time
+    form with host synchronisation after every operator, then Poisson
+    arrivals at fractions of the saturation rate for p50/p99 latency."""
+    from redfox_amd import serving as S
+
+    free, _ = mem_info(rf)
+    srv0 = S.Server(0, slots=slots, budget_bytes=1 << 30)
+    cat = S.Catalog(ctx, srv0, per_kind=4, exp=(5.0, 8.0))
+    free, _ = mem_info(rf)
+    budget = int(min(200 << 30, free - (16 << 30)))
+    srv = S.Server(0, slots=slots, budget_bytes=budget)
+    cat.register(srv)
+    srv0.close()
+    warm = S.make_requests(cat, 32, rate=None, seed=1)
+    srv.run(warm)
+    reqs = S.make_requests(cat, n_req, rate=None, seed=2)
+    st = srv.run(reqs)
+    sat = S.latency_summary(reqs, st)
+    plain = S.copy_requests(reqs)
+    stp = srv.run(plain, S.MODE_PLAIN)
+    pl = S.latency_summary(plain, stp)
+    same = [S.outcome(r) for r in reqs] == [S.outcome(r) for r in plain]
+    ok = all(r.status == 0 for r in reqs)
+    rows = sum(_request_rows(cat, r) for r in reqs)
+    sweep = []
+    for f in rates:
+        # The same requests, arriving as a Poisson process of rate lam.
+        lam = f * sat["qps"]
+        rq = S.make_requests(cat, n_req, rate=lam, seed=2)
+        st2 = srv.run(rq)
+        d = S.latency_summary(rq, st2)
+        d["lambda"] = lam
+        same = same and [S.outcome(r) for r in rq] == [S.outcome(r) for r in plain]
+        sweep.append(d)
+    srv.close()
+    return {"catalog_relations_GiB": cat.bytes / 2**30, "budget_GiB": budget / 2**30,
+            "slots": slots, "requests": n_req, "input_rows_per_batch": rows,
+            "saturation": sat, "plain": pl, "speedup_qps": sat["qps"] / pl["qps"],
+            "results_identical_in_every_run": bool(same and ok), "poisson": sweep}
+
+
+def _request_rows(cat, r):
+    from redfox_amd import serving as S
+
+    p = S.plan_of(cat, r.schema)
+    sch = next(s for s in cat.schemas[p] if s["id"] == r.schema)
+    return int(sum(sch["sizes"]))
+
+
+def mem_info(rf):
+    import ctypes
+
+    f, t = ctypes.c_uint64(), ctypes.c_uint64()
+    rf.lib().rfx_mem_info(ctypes.byref(f), ctypes.byref(t))
+    return int(f.value), int(t.value)
+
+
 def cpu_baseline_sort():
     """The oracle's serial LSD sort (one thread) on 2^26 of the 2^28 pairs."""
     import oracle
@@ -303,6 +364,8 @@ def main():
         ops["q1_60M"] = bench_q1(rf, ctx)
         log("batched 512 join-aggregate instances ...")
         ops["batched_512"] = bench_batched(rf, ctx)
+        log("served mixed requests ...")
+        ops["served_mix"] = bench_served(rf, ctx)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")

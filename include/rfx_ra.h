@@ -293,6 +293,12 @@ int rfx_project(rfx_ctx* ctx, void* out, int out_bytes, const void* in,
                 int in_bytes, uint64_t n, int nfields,
                 const uint32_t* in_shift, const uint32_t* bits,
                 const uint32_t* out_shift);
+/* rfx_project over a device row count: min(*n, n_cap) rows (the size
+ * variable a preceding select wrote, as Harmony passes it). */
+int rfx_project_sized(rfx_ctx* ctx, void* out, int out_bytes, const void* in,
+                      int in_bytes, const uint64_t* n, uint64_t n_cap,
+                      int nfields, const uint32_t* in_shift,
+                      const uint32_t* bits, const uint32_t* out_shift);
 /* split (Split_Merge.h:303-330): key = in >> key_shift, value = low
  * key_shift bits.  val may be NULL (split_key). */
 int rfx_split(rfx_ctx* ctx, void* key, int key_bytes, void* val,
@@ -315,6 +321,95 @@ int rfx_q1(rfx_ctx* ctx, double* out, const int32_t* shipdate,
            const uint8_t* returnflag, const uint8_t* linestatus,
            const int32_t* qty, const double* price, const double* disc,
            const double* tax, uint64_t n, int32_t cutoff);
+
+/* ---------------------------------------------------------------------- */
+/* Served form: a request loop over device-resident relations.            */
+/* ---------------------------------------------------------------------- */
+/* No single counterpart in the reference: it replaces Harmony's loop over
+ * one plan's nodes (Runtime.cpp:303-445, 506-646, 1054-1314) with a loop
+ * over many plans' requests.  Plain mode runs one request at a time and,
+ * like Harmony, synchronises after every operator and reads each size
+ * variable back; continuous mode admits requests in arrival order whenever
+ * one of `slots` streams and enough of the HBM budget are free, enqueues
+ * each request's whole plan on its slot with sizes kept on the device, and
+ * polls one event per slot, so newly arrived requests run beside those
+ * already on the GPU.  Every request's intermediates, outputs and operator
+ * workspace come from one block of the budget, freed when it completes. */
+typedef struct rfx_server rfx_server;
+
+/* Plans (the relations each takes, in order):
+ * SELECT_PROJECT: T, 16-byte tuples [127:96] id, [95:64] qty, [63:0] val;
+ *   SELECT id, val FROM T WHERE qty <param> constant (param = RFX_CMP_*,
+ *   0 meaning RFX_CMP_LT); output 8-byte tuples id << 32 | (u32)val.
+ * JOIN_AGG: F (fk u32, m f64), D (dkey u32 unique, grp u32 of val_bits
+ *   bits); SELECT grp, SUM(m), COUNT(*) FROM F JOIN D ON fk = dkey GROUP BY
+ *   grp; outputs keys u32, sums f64, counts u32.
+ * JOIN3: F (fk1 u32, m f64), S (skey u32 unique, fk2 u32 of val_bits
+ *   bits), T (tkey u32 unique, tval u32); F JOIN S ON fk1 = skey JOIN T ON
+ *   fk2 = tkey; outputs (tval u32, m f64) in join order.
+ * SET_OP: A, B sorted unique u64 (unsigned order); A <param> B with param
+ *   = RFX_SET_*; output u64.
+ * The join plans expect every left row to match exactly one right row
+ * (foreign keys into a primary key); a request that breaks this ends with
+ * status RFX_EINVAL. */
+#define RFX_PLAN_SELECT_PROJECT 0
+#define RFX_PLAN_JOIN_AGG 1
+#define RFX_PLAN_JOIN3 2
+#define RFX_PLAN_SET_OP 3
+
+typedef struct rfx_relation {
+  const void* col[2];   /* key column, value column (device) */
+  uint64_t rows;
+  uint32_t key_bits;    /* significant bits of the key column (radix passes) */
+  uint32_t val_bits;    /* significant bits of the value column, where a plan
+                           sorts or groups by it */
+} rfx_relation;
+
+typedef struct rfx_request {
+  uint32_t schema;      /* from rfx_server_add_schema */
+  uint32_t param;       /* RFX_CMP_* (select) or RFX_SET_* (set op) */
+  uint64_t constant;    /* select constant */
+  uint64_t arrival_ns;  /* arrival time after the start of the run */
+  void* sink;           /* optional device buffer receiving the output
+                           columns, each at capacity rows, in order */
+  uint64_t sink_bytes;
+  /* filled in by rfx_server_run */
+  uint64_t admit_ns, done_ns;  /* since the start of the run */
+  uint64_t mem_bytes;   /* bytes of the budget the request held */
+  uint64_t rows;        /* output rows */
+  uint64_t digest[3];   /* per output column: sum_i fmix64(w_i +
+                           fmix64(0x9E3779B97F4A7C15 * (c + 1) + i)) */
+  int32_t status;       /* RFX_OK, or the request's error */
+  uint32_t reserved;
+} rfx_request;
+
+typedef struct rfx_serve_stats {
+  uint64_t wall_ns;          /* first admission window to last completion */
+  uint64_t peak_bytes;       /* most budget bytes held at once */
+  uint64_t budget_bytes;
+  uint64_t blocked_on_memory;  /* admissions deferred for lack of budget */
+  uint32_t peak_inflight;    /* most requests on the GPU at once */
+  uint32_t slots;
+} rfx_serve_stats;
+
+#define RFX_SERVE_PLAIN 0
+#define RFX_SERVE_CONTINUOUS 1
+
+/* A server on `device` with `slots` streams (1..64) and an HBM budget of
+ * budget_bytes for requests (allocated once, up front). */
+int rfx_server_create(rfx_server** srv, int device, int slots,
+                      uint64_t budget_bytes);
+int rfx_server_destroy(rfx_server* srv);
+/* Register resident relations for one plan; *schema_id names them. */
+int rfx_server_add_schema(rfx_server* srv, int plan, const rfx_relation* rels,
+                          int nrels, uint32_t* schema_id);
+/* Budget bytes a request needs (its whole plan, workspace included). */
+int rfx_server_request_bytes(rfx_server* srv, const rfx_request* req,
+                             uint64_t* bytes);
+/* Serve n requests (arrival_ns non-decreasing) and return when all are
+ * done.  stats may be NULL. */
+int rfx_server_run(rfx_server* srv, rfx_request* reqs, uint64_t n, int mode,
+                   rfx_serve_stats* stats);
 
 /* ---------------------------------------------------------------------- */
 /* Harmony-convention entry points (synchronous, default context).        */

@@ -80,6 +80,12 @@ SIGNATURES = {
     "rfx_unique": (I, [P, P, P, P, U64, I]),
     "rfx_select": (I, [P, P, P, P, U64, I, U32, U32, I, U64]),
     "rfx_project": (I, [P, P, I, P, I, U64, I, P, P, P]),
+    "rfx_project_sized": (I, [P, P, I, P, I, P, U64, I, P, P, P]),
+    "rfx_server_create": (I, [POINTER(c_void_p), I, I, U64]),
+    "rfx_server_destroy": (I, [P]),
+    "rfx_server_add_schema": (I, [P, I, P, I, POINTER(c_uint32)]),
+    "rfx_server_request_bytes": (I, [P, P, U64P]),
+    "rfx_server_run": (I, [P, P, U64, I, P]),
     "rfx_split": (I, [P, P, I, P, I, P, I, U64, U32]),
     "rfx_merge": (I, [P, P, I, P, I, P, I, U64, U32]),
     "rfx_generate_ones": (I, [P, P, U64]),

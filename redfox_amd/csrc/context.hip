@@ -30,6 +30,10 @@ int ensure_workspace(rfx_ctx* ctx, uint64_t bytes) {
   if (ctx->capturing)
     return fail(RFX_EINVAL, "workspace must grow to %llu bytes during graph capture: reserve first",
                 static_cast<unsigned long long>(bytes));
+  if (ctx->ws_borrowed)
+    return fail(RFX_EBUDGET, "operator needs %llu workspace bytes, the borrowed region holds %llu",
+                static_cast<unsigned long long>(bytes),
+                static_cast<unsigned long long>(ctx->ws_bytes));
   RFX_HIP(hipSetDevice(ctx->device));
   if (ctx->ws) {
     // Work already queued may still read the old workspace.
@@ -242,7 +246,7 @@ int rfx_ctx_destroy(rfx_ctx* ctx) {
   if (!ctx) return RFX_OK;
   (void)prof_collect(ctx);
   for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
-  if (ctx->ws) {
+  if (ctx->ws && !ctx->ws_borrowed) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(ctx->ws);
   }

@@ -25,6 +25,7 @@ struct rfx_ctx {
   // Per-launch timing (off unless rfx_ctx_profile enabled it).
   bool profiling = false;
   bool capturing = false;     // inside rfx_graph_begin/end: no allocation
+  bool ws_borrowed = false;   // ws belongs to the caller: never grown or freed
   std::vector<rfx_prof_rec> prof_pending;
   std::vector<hipEvent_t> prof_pool;
   std::map<std::string, std::pair<uint64_t, double>> prof_stats;  // launches, ms

@@ -142,8 +142,11 @@ struct ProjSpec {
   uint32_t in_shift[16], bits[16], out_shift[16];
 };
 
+// n_dev (may be null): a device row count; the rows processed are then
+// min(*n_dev, n), n being the capacity the grid was sized for.
 __global__ __launch_bounds__(256) void k_project(void* out, int ob, const void* in, int ib,
-                                                 uint64_t n, ProjSpec ps) {
+                                                 uint64_t n, const uint64_t* n_dev, ProjSpec ps) {
+  if (n_dev && *n_dev < n) n = *n_dev;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += stride) {
     const u128 t = load_any(in, i, ib);
@@ -231,9 +234,11 @@ extern "C" int rfx_select(rfx_ctx* ctx, void* out, uint64_t* n_out, const void* 
   return fail(RFX_EINVAL, "rfx_select: tuple width %d", tuple_bytes);
 }
 
-extern "C" int rfx_project(rfx_ctx* ctx, void* out, int out_bytes, const void* in, int in_bytes,
-                           uint64_t n, int nfields, const uint32_t* in_shift,
-                           const uint32_t* bits, const uint32_t* out_shift) {
+namespace {
+
+int project_impl(rfx_ctx* ctx, void* out, int out_bytes, const void* in, int in_bytes, uint64_t n,
+                 const uint64_t* n_dev, int nfields, const uint32_t* in_shift,
+                 const uint32_t* bits, const uint32_t* out_shift) {
   if (!ctx || (n && (!out || !in))) return fail(RFX_EINVAL, "rfx_project: null argument");
   if (!valid_width(out_bytes) || !valid_width(in_bytes))
     return fail(RFX_EINVAL, "rfx_project: widths %d/%d", out_bytes, in_bytes);
@@ -250,9 +255,28 @@ extern "C" int rfx_project(rfx_ctx* ctx, void* out, int out_bytes, const void* i
     ps.out_shift[f] = out_shift[f];
   }
   if (!n) return RFX_OK;
-  k_project<<<stream_grid(ctx, n), 256, 0, ctx->stream>>>(out, out_bytes, in, in_bytes, n, ps);
+  k_project<<<stream_grid(ctx, n), 256, 0, ctx->stream>>>(out, out_bytes, in, in_bytes, n, n_dev,
+                                                          ps);
   RFX_LAUNCHED();
   return RFX_OK;
+}
+
+}  // namespace
+
+extern "C" int rfx_project(rfx_ctx* ctx, void* out, int out_bytes, const void* in, int in_bytes,
+                           uint64_t n, int nfields, const uint32_t* in_shift,
+                           const uint32_t* bits, const uint32_t* out_shift) {
+  return project_impl(ctx, out, out_bytes, in, in_bytes, n, nullptr, nfields, in_shift, bits,
+                      out_shift);
+}
+
+extern "C" int rfx_project_sized(rfx_ctx* ctx, void* out, int out_bytes, const void* in,
+                                 int in_bytes, const uint64_t* n, uint64_t n_cap, int nfields,
+                                 const uint32_t* in_shift, const uint32_t* bits,
+                                 const uint32_t* out_shift) {
+  if (n_cap && !n) return fail(RFX_EINVAL, "rfx_project_sized: null row count");
+  return project_impl(ctx, out, out_bytes, in, in_bytes, n_cap, n, nfields, in_shift, bits,
+                      out_shift);
 }
 
 extern "C" int rfx_split(rfx_ctx* ctx, void* key, int key_bytes, void* val, int val_bytes,
