@@ -215,6 +215,101 @@ def bench_q1(rf, ctx, iters=5):
             "kept_rows": int(res[:, 5].sum())}
 
 
+def bench_agg(rf, ctx, iters=5):
+    """GROUP BY key: SUM(v), COUNT(*) over 2^28 rows sorted by key (u32
+    keys Zipf(1.1) over 2^20 values, f64 values) in one reduce-by-key pass."""
+    n, groups = 1 << 28, 1 << 20
+    keys, vals = rf.DeviceBuffer(4 * n), rf.DeviceBuffer(8 * n)
+    ident = rf.DeviceBuffer(4 * groups)
+    ctx.iota_u32(ident, groups)
+    row_off = rf.DeviceBuffer.from_numpy(np.array([0, n], dtype=np.uint64))
+    dim_off = rf.DeviceBuffer.from_numpy(np.array([0, groups], dtype=np.uint64))
+    ctx.gen_fk_batch(keys, vals, row_off, dim_off, ident, 1, n, SEED + 5, 1.1)
+    ctx.sort_pairs(keys, vals, n, 4, 20, 8)
+    ok, ov, oc = rf.DeviceBuffer(4 * groups), rf.DeviceBuffer(8 * groups), rf.DeviceBuffer(4 * groups)
+    nout, ncell = rf.u64_cell(), rf.u64_cell(n)
+    ctx.synchronize()
+    ctx.profile(True)
+    times = []
+    for _ in range(iters + 1):
+        t = rf.Timer(ctx)
+        t.begin()
+        ctx.reduce_count_by_key_f64(ok, ov, oc, nout, keys, vals, ncell, n, 4, 0)
+        times.append(t.end())
+        t.close()
+    launches, ms = ctx.profile_read("rbk")
+    ctx.profile(False)
+    g = rf.read_u64(nout)
+    cnt = oc.download(np.uint32, g)
+    res = {"n": n, "groups": g, "ms": median(times[1:]), "rows_per_s": n / (median(times[1:]) * 1e-3),
+           "counts_sum_to_n": bool(int(cnt.sum(dtype=np.uint64)) == n),
+           "roofline": roofline(12 * n + 16 * g, launches, ms)}
+    for b in (keys, vals, ident, row_off, dim_off, ok, ov, oc, nout, ncell):
+        b.close()
+    return res
+
+
+def bench_select(rf, ctx, iters=5):
+    """SELECT on 2^27 16-byte tuples, a 32-bit field < 2^31 (keeps half)."""
+    n = 1 << 27
+    hi, lo = rf.DeviceBuffer(8 * n), rf.DeviceBuffer(8 * n)
+    ctx.gen_u64(hi, n, SEED + 6)
+    ctx.gen_u64(lo, n, SEED + 7)
+    tup, out = rf.DeviceBuffer(16 * n), rf.DeviceBuffer(16 * n)
+    ctx.merge(tup, 16, hi, 8, lo, 8, n, 64)
+    nout = rf.u64_cell()
+    ctx.synchronize()
+    hi.close()
+    lo.close()
+    ctx.profile(True)
+    times = []
+    for _ in range(iters + 1):
+        t = rf.Timer(ctx)
+        t.begin()
+        ctx.select(out, nout, tup, n, 16, 64, 32, 2, 1 << 31)
+        times.append(t.end())
+        t.close()
+    launches, ms = ctx.profile_read("select")
+    ctx.profile(False)
+    kept = rf.read_u64(nout)
+    res = {"n": n, "kept": kept, "ms": median(times[1:]), "rows_per_s": n / (median(times[1:]) * 1e-3),
+           "roofline": roofline(16 * n + 16 * kept, launches, ms)}
+    for b in (tup, out, nout):
+        b.close()
+    return res
+
+
+def bench_setop(rf, ctx, iters=5):
+    """Union, intersection and difference of two sorted unique u64 sets of
+    2^27 elements each (half of B inside A)."""
+    from redfox_amd import serving
+
+    n = 1 << 27
+    a, na = serving.sorted_set_u64(ctx, n, SEED + 8)
+    b, nb = serving.sorted_set_u64(ctx, n, SEED + 8, first=n // 2)
+    out = rf.DeviceBuffer(8 * (na + nb))
+    nout = rf.u64_cell()
+    res = {"na": na, "nb": nb}
+    for op, name in ((0, "union"), (2, "intersection"), (1, "difference")):
+        ctx.profile(True)
+        times = []
+        for _ in range(iters + 1):
+            t = rf.Timer(ctx)
+            t.begin()
+            ctx.set_op(op, out, nout, a, na, b, nb, 8)
+            times.append(t.end())
+            t.close()
+        launches, ms = ctx.profile_read("setop")
+        ctx.profile(False)
+        m = rf.read_u64(nout)
+        res[name] = {"out": m, "ms": median(times[1:]),
+                     "rows_per_s": (na + nb) / (median(times[1:]) * 1e-3),
+                     "roofline": roofline(8 * (na + nb + m), launches, ms)}
+    for x in (a, b, out, nout):
+        x.close()
+    return res
+
+
 def bench_batched(rf, ctx, reps=5):
     """512 join + group-aggregate instances (fact 10^U[3,7] rows, Zipf(1.1)
     foreign keys into dimensions of 10^U[2,5] rows): the plain per-operator
@@ -362,6 +457,12 @@ def main():
         ops["join_2^24x2^26"] = bench_join(rf, ctx, 1 << 24, 1 << 26, 24, 5)
         log("q1 60M ...")
         ops["q1_60M"] = bench_q1(rf, ctx)
+        log("group-by sum+count 2^28 ...")
+        ops["agg_2^28"] = bench_agg(rf, ctx)
+        log("select 2^27 16-byte tuples ...")
+        ops["select_2^27"] = bench_select(rf, ctx)
+        log("set operations 2^27 + 2^27 u64 ...")
+        ops["setop_2^27"] = bench_setop(rf, ctx)
         log("batched 512 join-aggregate instances ...")
         ops["batched_512"] = bench_batched(rf, ctx)
         log("served mixed requests ...")

@@ -72,6 +72,7 @@ SIGNATURES = {
     "rfx_join_expand_project": (I, [P, P, P, U64, P, U64, P, I, P, I, P, I, P, I, P, I, I, P]),
     "rfx_reduce_by_key_f64": (I, [P, P, P, P, P, P, P, U64, I, I]),
     "rfx_reduce_by_key_u32": (I, [P, P, P, P, P, P, P, U64, I]),
+    "rfx_reduce_count_by_key_f64": (I, [P, P, P, P, P, P, P, P, U64, I, I]),
     "rfx_count_by_key": (I, [P, P, P, P, P, P, U64, I]),
     "rfx_total_f64": (I, [P, P, P, U64]),
     "rfx_total_u32": (I, [P, P, P, U64]),

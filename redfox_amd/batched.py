@@ -14,8 +14,8 @@ with dim_i = (dkey: a permutation of 0..d_i-1, grp: U[0, 64)) and fact_i =
   the next operator (Runtime.cpp:99-118, 1054-1314);
 * batched: all instances as one plan.  Keys are prefixed with the instance
   number (rfx_segment_keys), so one radix sort, one find_bounds, one
-  expansion, one regrouping sort and one reduce-by-key cover every
-  instance.  The whole sequence is captured once into a HIP graph and
+  expansion, one regrouping sort and one pass of reduce-by-key (sum and
+  count together) cover every instance.  The whole sequence is captured once into a HIP graph and
   replayed with a single launch.  Stable sorts keep each instance's rows in
   the order the plain form sees them, so keys and counts are identical and
   FP64 sums agree to rounding.
@@ -169,8 +169,7 @@ class BatchedRunner:
         self.key2, self.val = B(4 * N), B(8 * N)
         cap = GROUPS * w.n_inst
         self.okeys, self.osum, self.ocnt = B(4 * cap), B(8 * cap), B(4 * cap)
-        self.okeys2 = B(4 * cap)
-        self.nout, self.nout2 = R.u64_cell(), R.u64_cell()
+        self.nout = R.u64_cell()
         self.seg_bits = max(1, int(np.ceil(np.log2(w.n_inst))))
         self.graph = None
 
@@ -193,8 +192,8 @@ class BatchedRunner:
                                (0, 2, 0, GRP_BITS, 0),                    # grp
                                (1, 1, 0, 64, 0)])                         # measure
         c.sort_pairs(self.key2, self.val, N, 4, self.seg_bits + GRP_BITS, 8)
-        c.reduce_by_key_f64(self.okeys, self.osum, self.nout, self.key2, self.val, self.tot, N, 4, 0)
-        c.count_by_key(self.okeys2, self.ocnt, self.nout2, self.key2, self.tot, N, 4)
+        c.reduce_count_by_key_f64(self.okeys, self.osum, self.ocnt, self.nout, self.key2, self.val,
+                                  self.tot, N, 4, 0)
 
     def capture(self) -> None:
         """Run once eagerly (sizes the workspace), then record the graph."""

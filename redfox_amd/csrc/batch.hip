@@ -27,13 +27,24 @@ __device__ __forceinline__ uint32_t segment_of(const uint64_t* offsets, uint32_t
   return lo;
 }
 
-__global__ void k_segment_keys(uint32_t* __restrict__ out, const uint32_t* __restrict__ key,
-                               const uint64_t* __restrict__ offsets, uint32_t nseg, uint64_t n,
-                               uint32_t key_bits) {
+constexpr uint32_t kSegLds = 4096;  // segment offsets searched in LDS
+
+__global__ __launch_bounds__(256) void k_segment_keys(uint32_t* __restrict__ out,
+                                                      const uint32_t* __restrict__ key,
+                                                      const uint64_t* __restrict__ offsets,
+                                                      uint32_t nseg, uint64_t n,
+                                                      uint32_t key_bits) {
+  __shared__ uint64_t s_off[kSegLds + 1];
+  const bool lds = nseg <= kSegLds;
+  if (lds) {
+    for (uint32_t i = threadIdx.x; i <= nseg; i += blockDim.x) s_off[i] = offsets[i];
+    __syncthreads();
+  }
+  const uint64_t* off = lds ? s_off : offsets;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
        i += stride) {
-    const uint32_t s = segment_of(offsets, nseg, i);
+    const uint32_t s = segment_of(off, nseg, i);
     out[i] = (s << key_bits) | key[i];
   }
 }

@@ -22,9 +22,6 @@
 namespace rfx {
 namespace {
 
-constexpr int kJoinNT = 256;
-constexpr int kJoinVT = 8;
-constexpr int kJoinNV = kJoinNT * kJoinVT;
 
 // ---- element access by runtime width -------------------------------------
 __device__ __forceinline__ u128 load_w(const void* base, uint64_t i, int w) {
@@ -64,28 +61,43 @@ __device__ __forceinline__ uint64_t merge_path_lower(const K* a, uint64_t na, co
 // ---------------------------------------------------------------------------
 // find_bounds
 // ---------------------------------------------------------------------------
+#ifndef RFX_FB_NV
+#define RFX_FB_NV 4096
+#endif
+// find_bounds tile: merged (left + right) elements per block.
+constexpr int kFbNV = RFX_FB_NV;
+constexpr int kFbNT = 512;
+
 template <class K>
 __global__ __launch_bounds__(256) void k_fb_partition(const K* __restrict__ a, uint64_t na,
                                                       const K* __restrict__ b, uint64_t nb,
-                                                      uint64_t ntiles,
+                                                      uint64_t ntiles, uint32_t nv,
                                                       uint64_t* __restrict__ split) {
   uint64_t t = blockIdx.x * 256ull + threadIdx.x;
   if (t > ntiles) return;
-  uint64_t diag = t * kJoinNV;
+  uint64_t diag = t * nv;
   if (diag > na + nb) diag = na + nb;
   split[t] = merge_path_lower(a, na, b, nb, diag);
 }
 
-template <class K>
-__global__ __launch_bounds__(kJoinNT) void k_find_bounds(
+// One tile of the merge path: its left keys A[a0, a1) and right keys
+// B[b0, b1).  With A winning ties, every B[j] before b0 is < A[a0] and every
+// B[j] from b1 on is >= A[a1 - 1], so each left key's lower bound lies in
+// [b0, b1] and is found by a binary search of the tile's right keys in LDS
+// (one left key per lane: neighbouring lanes probe the same words), and so
+// is its upper bound; the equal-key count is their difference, extended by
+// a gallop through global memory only for a run that crosses the tile's
+// end.
+template <class K, int NT, int NV>
+__global__ __launch_bounds__(NT) void k_find_bounds(
     const K* __restrict__ a, uint64_t na, const K* __restrict__ b, uint64_t nb,
     const uint64_t* __restrict__ split, uint32_t ntiles, int32_t* __restrict__ lower_bound,
     int32_t* __restrict__ left_count, uint64_t* __restrict__ total,
     uint64_t* __restrict__ status, uint32_t* __restrict__ ticket) {
-  __shared__ K keys[kJoinNV];
-  __shared__ int32_t s_lb[kJoinNV];
-  __shared__ int32_t s_cnt[kJoinNV];
-  __shared__ uint64_t scan_tmp[kJoinNT / kWave + 1];
+  constexpr int VT = NV / NT;
+  __shared__ K keys[NV];
+  __shared__ uint32_t s_cnt[NV];
+  __shared__ uint64_t scan_tmp[NT / kWave + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
 
@@ -93,66 +105,103 @@ __global__ __launch_bounds__(kJoinNT) void k_find_bounds(
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
-  const uint64_t d0 = static_cast<uint64_t>(tile) * kJoinNV;
-  const uint64_t d1 = (d0 + kJoinNV < na + nb) ? d0 + kJoinNV : na + nb;
+  const uint64_t d0 = static_cast<uint64_t>(tile) * NV;
+  const uint64_t d1 = (d0 + NV < na + nb) ? d0 + NV : na + nb;
   const uint64_t a0 = split[tile], a1 = split[tile + 1];
   const uint64_t b0 = d0 - a0, b1 = d1 - a1;
   const uint32_t nat = static_cast<uint32_t>(a1 - a0), nbt = static_cast<uint32_t>(b1 - b0);
 
-  // Tile's A keys then B keys, contiguous in LDS.
-  for (uint32_t i = tid; i < nat + nbt; i += kJoinNT)
-    keys[i] = i < nat ? a[a0 + i] : b[b0 + (i - nat)];
+  // Tile's A keys then B keys, contiguous in LDS; all loads in flight
+  // before the first LDS write.
+  if constexpr (sizeof(K) <= 8) {
+    K v[VT];
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      const uint32_t i = tid + j * NT;
+      if (i < nat) v[j] = a[a0 + i];
+      else if (i < nat + nbt) v[j] = b[b0 + (i - nat)];
+    }
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      const uint32_t i = tid + j * NT;
+      if (i < nat + nbt) keys[i] = v[j];
+    }
+  } else {
+    for (uint32_t i = tid; i < nat + nbt; i += NT)
+      keys[i] = i < nat ? a[a0 + i] : b[b0 + (i - nat)];
+  }
   __syncthreads();
   const K* sa = keys;
   const K* sb = keys + nat;
 
-  // Thread's slice of the tile's merge path.
-  const uint32_t diag = tid * kJoinVT;
-  uint32_t ai = 0, bj = 0;
-  if (diag < nat + nbt) {
-    ai = static_cast<uint32_t>(merge_path_lower(sa, nat, sb, nbt, diag));
-    bj = diag - ai;
-  }
-  uint64_t my_sum = 0;
-  const uint32_t a_begin = ai;
+  // Lower and upper bounds of each left key in the tile's right keys by
+  // fixed-step binary searches: all of a thread's searches advance in
+  // lockstep, so their LDS reads overlap.  JA (block-uniform) bounds the
+  // left keys per thread.
+  const uint32_t JA = (nat + NT - 1) / NT;
+  uint32_t top = 0;
+  if (nbt) top = 1u << (31 - __builtin_clz(nbt));
+  K kv[VT];
+  uint32_t lo[VT], up[VT];
 #pragma unroll
-  for (int s = 0; s < kJoinVT; ++s) {
-    if (diag + s >= nat + nbt) break;
-    const bool take_a = ai < nat && (bj >= nbt || !lt(sb[bj], sa[ai]));
-    if (take_a) {
-      const K key = sa[ai];
-      const uint64_t lb = b0 + bj;
-      // Gallop for the first right key > key, starting at lb.
-      uint64_t cnt = 0;
-      auto rk = [&](uint64_t idx) -> K { return idx < b1 ? sb[idx - b0] : b[idx]; };
-      if (lb < nb && eq(rk(lb), key)) {
-        uint64_t lo = lb + 1, step = 1, hi = lb + 1;
-        while (true) {  // invariant: rk(lo - 1) == key
-          hi = lo + step - 1;
-          if (hi >= nb) { hi = nb; break; }
-          if (!eq(rk(hi), key)) break;
-          lo = hi + 1;
-          step <<= 1;
-        }
-        while (lo < hi) {  // first index in [lo, hi) whose key differs
-          uint64_t mid = (lo + hi) >> 1;
-          if (eq(rk(mid), key)) lo = mid + 1;
-          else hi = mid;
-        }
-        cnt = lo - lb;
-      }
-      s_lb[ai] = static_cast<int32_t>(lb);
-      s_cnt[ai] = static_cast<int32_t>(cnt);
-      my_sum += cnt;
-      ++ai;
-    } else {
-      ++bj;
+  for (int j = 0; j < VT; ++j) {
+    lo[j] = up[j] = 0;
+    if (static_cast<uint32_t>(j) < JA) {
+      const uint32_t i = tid + j * NT;
+      kv[j] = sa[i < nat ? i : nat - 1];
     }
   }
-  const uint32_t a_end = ai;
+  for (uint32_t step = top; step; step >>= 1) {
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      if (static_cast<uint32_t>(j) < JA) {
+        if (lo[j] + step <= nbt && lt(sb[lo[j] + step - 1], kv[j])) lo[j] += step;
+        if (up[j] + step <= nbt && !lt(kv[j], sb[up[j] + step - 1])) up[j] += step;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < VT; ++j) {
+    const uint32_t i = tid + j * NT;
+    if (static_cast<uint32_t>(j) < JA && i < nat) {
+      const K key = kv[j];
+      uint64_t cnt = up[j] - lo[j];
+      // A run of equal right keys that reaches the tile's end continues
+      // in global memory from b1.
+      if (up[j] == nbt && b1 < nb && eq(b[b1], key)) {
+        uint64_t l = b1 + 1, step = 1, h;
+        while (true) {  // invariant: b[l - 1] == key
+          h = l + step - 1;
+          if (h >= nb) { h = nb; break; }
+          if (!eq(b[h], key)) break;
+          l = h + 1;
+          step <<= 1;
+        }
+        while (l < h) {  // first index in [l, h) whose key differs
+          const uint64_t mid = (l + h) >> 1;
+          if (eq(b[mid], key)) l = mid + 1;
+          else h = mid;
+        }
+        cnt += l - b1;
+      }
+      lower_bound[a0 + i] = static_cast<int32_t>(b0 + lo[j]);
+      s_cnt[i] = static_cast<uint32_t>(cnt);
+    }
+  }
+  __syncthreads();
 
+  // Exclusive scan of the counts in left-key order: thread t owns
+  // elements [t*VT, t*VT + VT).
+  const uint32_t c0 = tid * VT;
+  uint32_t cv[VT];
+  uint64_t my_sum = 0;
+#pragma unroll
+  for (int j = 0; j < VT; ++j) {
+    cv[j] = (c0 + j < nat) ? s_cnt[c0 + j] : 0u;
+    my_sum += cv[j];
+  }
   uint64_t tile_sum;
-  uint64_t my_prefix = block_excl_scan<kJoinNT>(my_sum, scan_tmp, &tile_sum);
+  uint64_t run = block_excl_scan<NT>(my_sum, scan_tmp, &tile_sum);
 
   // Tile prefix by decoupled look-back (wave 0).
   if (tid < kWave) {
@@ -168,19 +217,15 @@ __global__ __launch_bounds__(kJoinNT) void k_find_bounds(
       }
     }
   }
-  // Exclusive prefix of each A element: overwrite its count in LDS.
-  uint64_t run = my_prefix;
-  for (uint32_t i = a_begin; i < a_end; ++i) {
-    const uint64_t c = static_cast<uint64_t>(static_cast<uint32_t>(s_cnt[i]));
-    s_cnt[i] = static_cast<int32_t>(run);  // tile-local prefix for now
-    run += c;
+#pragma unroll
+  for (int j = 0; j < VT; ++j) {
+    if (c0 + j < nat) s_cnt[c0 + j] = static_cast<uint32_t>(run);  // tile-local prefix
+    run += cv[j];
   }
   __syncthreads();
   const uint64_t prefix = s_prefix;
-  for (uint32_t i = tid; i < nat; i += kJoinNT) {
-    lower_bound[a0 + i] = s_lb[i];
-    left_count[a0 + i] = static_cast<int32_t>(prefix + static_cast<uint32_t>(s_cnt[i]));
-  }
+  for (uint32_t i = tid; i < nat; i += NT)
+    left_count[a0 + i] = static_cast<int32_t>(prefix + s_cnt[i]);
   if (tile == ntiles - 1 && tid == 0) *total = prefix + tile_sum;
 }
 
@@ -206,12 +251,13 @@ __global__ __launch_bounds__(256) void k_expand_partition(const int32_t* __restr
                                                           uint64_t nrows,
                                                           const uint64_t* __restrict__ total_p,
                                                           uint64_t total_cap, uint64_t ntiles_cap,
+                                                          uint32_t nv,
                                                           uint64_t* __restrict__ split) {
   uint64_t t = blockIdx.x * 256ull + threadIdx.x;
   if (t > ntiles_cap) return;
   // Never expand past the caller's capacity, whatever the device says.
   const uint64_t total = *total_p < total_cap ? *total_p : total_cap;
-  uint64_t diag = t * kJoinNV;
+  uint64_t diag = t * nv;
   if (diag > total + nrows) diag = total + nrows;
   split[t] = merge_path_rows(starts, nrows, total, diag);
 }
@@ -250,90 +296,101 @@ struct EmitSpec {
   void* out[2];
   int out_bytes[2];
   int nf;
+  uint32_t used;  // bit s: some field reads source s
   EmitField f[8];
 };
 
 __device__ __forceinline__ void emit(const EmitSpec& e, uint64_t g, uint64_t li, uint64_t ri) {
-  u128 o[2] = {{0, 0}, {0, 0}};
-  for (int k = 0; k < e.nf; ++k) {
-    const EmitField f = e.f[k];
-    const u128 v = load_w(e.src[f.src], f.src == 2 ? ri : li, e.src_bytes[f.src]);
-    o[f.col] = or128(o[f.col], shl128(mask128(shr128(v, f.in_shift), f.bits), f.out_shift));
+  const u128 zero{0, 0};
+  const u128 v0 = (e.used & 1) ? load_w(e.src[0], li, e.src_bytes[0]) : zero;
+  const u128 v1 = (e.used & 2) ? load_w(e.src[1], li, e.src_bytes[1]) : zero;
+  const u128 v2 = (e.used & 4) ? load_w(e.src[2], ri, e.src_bytes[2]) : zero;
+  u128 o0 = zero, o1 = zero;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (k < e.nf) {
+      const EmitField f = e.f[k];
+      const u128 v = f.src == 0 ? v0 : f.src == 1 ? v1 : v2;
+      const u128 x = shl128(mask128(shr128(v, f.in_shift), f.bits), f.out_shift);
+      if (f.col == 0) o0 = or128(o0, x);
+      else o1 = or128(o1, x);
+    }
   }
-  store_w(e.out[0], g, e.out_bytes[0], o[0]);
-  if (e.out[1]) store_w(e.out[1], g, e.out_bytes[1], o[1]);
+  store_w(e.out[0], g, e.out_bytes[0], o0);
+  if (e.out[1]) store_w(e.out[1], g, e.out_bytes[1], o1);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kJoinNT) void k_expand(
+#ifndef RFX_EX_NV
+#define RFX_EX_NV 4096
+#endif
+// expand tile: merged (output + row) items per block.
+constexpr int kExNV = RFX_EX_NV;
+constexpr int kExNT = 512;
+
+// One tile of the merge path of outputs against row starts: outputs
+// [g0, g1) and rows [r0, r1), plus row r0 - 1 (already open at the tile's
+// start) in LDS slot 0.  Output g belongs to slot #{tile rows j : start_j
+// <= g}, found by fixed-step binary searches of the slots' starts that a
+// thread runs in lockstep for all its outputs; its right index is the
+// row's lower bound plus g - start.  Outputs are thread-strided, so every
+// store is coalesced.
+template <int MODE, int NT, int NV>
+__global__ __launch_bounds__(NT) void k_expand(
     const int32_t* __restrict__ lower_bound, const int32_t* __restrict__ starts, uint64_t nrows,
     const uint64_t* __restrict__ total_p, uint64_t total_cap, const uint64_t* __restrict__ split,
     int32_t* __restrict__ left_idx, int32_t* __restrict__ right_idx, GatherSpec gs, EmitSpec es) {
-  __shared__ int32_t s_start[kJoinNV + 1];
-  __shared__ int32_t s_lb[kJoinNV + 1];
-  __shared__ int32_t s_li[kJoinNV];
-  __shared__ int32_t s_ri[kJoinNV];
+  constexpr int VT = NV / NT;
+  __shared__ uint32_t s_start[NV + 1];
+  __shared__ uint32_t s_lb[NV + 1];
 
   const uint64_t total = *total_p < total_cap ? *total_p : total_cap;
   const uint32_t tile = blockIdx.x;
-  const uint64_t d0 = static_cast<uint64_t>(tile) * kJoinNV;
+  const uint64_t d0 = static_cast<uint64_t>(tile) * NV;
   if (d0 >= total + nrows) return;
-  const uint64_t d1 = (d0 + kJoinNV < total + nrows) ? d0 + kJoinNV : total + nrows;
+  const uint64_t d1 = (d0 + NV < total + nrows) ? d0 + NV : total + nrows;
   const uint64_t g0 = split[tile], g1 = split[tile + 1];
   const uint64_t r0 = d0 - g0, r1 = d1 - g1;
   const uint32_t ng = static_cast<uint32_t>(g1 - g0), nr = static_cast<uint32_t>(r1 - r0);
   const uint32_t tid = threadIdx.x;
 
-  // Rows r0-1 .. r1-1: slot 0 holds the row already open at tile start.
-  for (uint32_t i = tid; i < nr + 1; i += kJoinNT) {
-    int64_t r = static_cast<int64_t>(r0) - 1 + i;
-    s_start[i] = r >= 0 ? starts[r] : 0;
-    s_lb[i] = r >= 0 ? lower_bound[r] : 0;
+  for (uint32_t i = tid; i < nr + 1; i += NT) {
+    const int64_t r = static_cast<int64_t>(r0) - 1 + i;
+    s_start[i] = r >= 0 ? static_cast<uint32_t>(starts[r]) : 0u;
+    s_lb[i] = r >= 0 ? static_cast<uint32_t>(lower_bound[r]) : 0u;
   }
   __syncthreads();
 
-  const uint32_t diag = tid * kJoinVT;
-  if (diag < ng + nr) {
-    // Local merge path over outputs g0.. and row starts s_start[1..nr].
-    uint32_t lo = diag > nr ? diag - nr : 0;
-    uint32_t hi = diag < ng ? diag : ng;
-    while (lo < hi) {
-      uint32_t mid = (lo + hi) >> 1;
-      uint64_t s = static_cast<uint32_t>(s_start[1 + diag - 1 - mid]);
-      if (g0 + mid < s) lo = mid + 1;
-      else hi = mid;
-    }
-    uint32_t gi = lo, rj = diag - lo;
+  const uint32_t JA = (ng + NT - 1) / NT;
+  uint32_t top = 0;
+  if (nr) top = 1u << (31 - __builtin_clz(nr));
+  uint32_t slot[VT];
 #pragma unroll
-    for (int s = 0; s < kJoinVT; ++s) {
-      if (diag + s >= ng + nr) break;
-      const bool take_row =
-          rj < nr && (gi >= ng || static_cast<uint64_t>(static_cast<uint32_t>(s_start[1 + rj])) <=
-                                      g0 + gi);
-      if (take_row) {
-        ++rj;
-      } else {
-        // Open row is local slot rj (row r0 - 1 + rj).
-        const uint64_t g = g0 + gi;
-        const int32_t row = static_cast<int32_t>(r0 - 1 + rj);
-        const int32_t rix = static_cast<int32_t>(
-            static_cast<uint32_t>(s_lb[rj]) + (g - static_cast<uint32_t>(s_start[rj])));
-        s_li[gi] = row;
-        s_ri[gi] = rix;
-        ++gi;
+  for (int j = 0; j < VT; ++j) slot[j] = 0;
+  for (uint32_t step = top; step; step >>= 1) {
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      if (static_cast<uint32_t>(j) < JA) {
+        const uint32_t g = static_cast<uint32_t>(g0) + tid + j * NT;
+        if (slot[j] + step <= nr && s_start[slot[j] + step] <= g) slot[j] += step;
       }
     }
   }
-  __syncthreads();
-  for (uint32_t i = tid; i < ng; i += kJoinNT) {
-    if constexpr (MODE == 0) {
-      left_idx[g0 + i] = s_li[i];
-      right_idx[g0 + i] = s_ri[i];
-    } else if constexpr (MODE == 2) {
-      emit(es, g0 + i, static_cast<uint32_t>(s_li[i]), static_cast<uint32_t>(s_ri[i]));
-    } else {
-      store_w(gs.out, g0 + i, gs.out_bytes,
-              compose(gs, static_cast<uint32_t>(s_li[i]), static_cast<uint32_t>(s_ri[i])));
+#pragma unroll
+  for (int j = 0; j < VT; ++j) {
+    const uint32_t i = tid + j * NT;
+    if (static_cast<uint32_t>(j) < JA && i < ng) {
+      const uint32_t g = static_cast<uint32_t>(g0) + i;
+      const uint32_t sl = slot[j];
+      const uint32_t row = static_cast<uint32_t>(r0 - 1 + sl);
+      const uint32_t rix = s_lb[sl] + (g - s_start[sl]);
+      if constexpr (MODE == 0) {
+        left_idx[g] = static_cast<int32_t>(row);
+        right_idx[g] = static_cast<int32_t>(rix);
+      } else if constexpr (MODE == 2) {
+        emit(es, g, row, rix);
+      } else {
+        store_w(gs.out, g, gs.out_bytes, compose(gs, row, rix));
+      }
     }
   }
 }
@@ -362,7 +419,8 @@ int find_bounds_impl(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, co
     RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(total), 2, 0));
     return RFX_OK;
   }
-  const uint64_t ntiles = (nl + nr + kJoinNV - 1) / kJoinNV;
+  constexpr int NV = sizeof(K) <= 8 ? kFbNV : 2048;
+  const uint64_t ntiles = (nl + nr + NV - 1) / NV;
   Carve c(nullptr);
   c.take<uint64_t>(ntiles + 1);
   c.take<uint64_t>(ntiles);
@@ -376,11 +434,11 @@ int find_bounds_impl(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, co
   RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(status),
                    (reinterpret_cast<char*>(ticket + 1) - reinterpret_cast<char*>(status)) / 4, 0));
   k_fb_partition<K><<<static_cast<uint32_t>((ntiles + 1 + 255) / 256), 256, 0, ctx->stream>>>(
-      static_cast<const K*>(left), nl, static_cast<const K*>(right), nr, ntiles, split);
+      static_cast<const K*>(left), nl, static_cast<const K*>(right), nr, ntiles, NV, split);
   RFX_LAUNCHED();
   {
     RFX_PROF(ctx, "join.find_bounds");
-    k_find_bounds<K><<<static_cast<uint32_t>(ntiles), kJoinNT, 0, ctx->stream>>>(
+    k_find_bounds<K, kFbNT, NV><<<static_cast<uint32_t>(ntiles), kFbNT, 0, ctx->stream>>>(
         static_cast<const K*>(left), nl, static_cast<const K*>(right), nr, split,
         static_cast<uint32_t>(ntiles), lb, lc, total, status, ticket);
     RFX_LAUNCHED();
@@ -392,24 +450,24 @@ int expand_impl(rfx_ctx* ctx, int mode, int32_t* li, int32_t* ri, const GatherSp
                 const uint64_t* total, uint64_t total_cap, const int32_t* lb, const int32_t* lc,
                 uint64_t nl, const EmitSpec& es = EmitSpec{}) {
   if (nl == 0 || total_cap == 0) return RFX_OK;
-  const uint64_t ntiles = (total_cap + nl + kJoinNV - 1) / kJoinNV;
+  const uint64_t ntiles = (total_cap + nl + kExNV - 1) / kExNV;
   Carve c(nullptr);
   c.take<uint64_t>(ntiles + 1);
   RFX_TRY(ensure_workspace(ctx, c.off));
   Carve w(ctx->ws);
   uint64_t* split = w.take<uint64_t>(ntiles + 1);
   k_expand_partition<<<static_cast<uint32_t>((ntiles + 1 + 255) / 256), 256, 0, ctx->stream>>>(
-      lc, nl, total, total_cap, ntiles, split);
+      lc, nl, total, total_cap, ntiles, kExNV, split);
   RFX_LAUNCHED();
   RFX_PROF(ctx, mode == 0 ? "join.expand" : mode == 1 ? "join.expand_gather" : "join.expand_project");
   if (mode == 0)
-    k_expand<0><<<static_cast<uint32_t>(ntiles), kJoinNT, 0, ctx->stream>>>(
+    k_expand<0, kExNT, kExNV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
         lb, lc, nl, total, total_cap, split, li, ri, gs, es);
   else if (mode == 1)
-    k_expand<1><<<static_cast<uint32_t>(ntiles), kJoinNT, 0, ctx->stream>>>(
+    k_expand<1, kExNT, kExNV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
         lb, lc, nl, total, total_cap, split, li, ri, gs, es);
   else
-    k_expand<2><<<static_cast<uint32_t>(ntiles), kJoinNT, 0, ctx->stream>>>(
+    k_expand<2, kExNT, kExNV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
         lb, lc, nl, total, total_cap, split, li, ri, gs, es);
   RFX_LAUNCHED();
   return RFX_OK;
@@ -525,6 +583,7 @@ extern "C" int rfx_join_expand_project(rfx_ctx* ctx, const int32_t* lower_bound,
   es.out[0] = out0; es.out[1] = out1;
   es.out_bytes[0] = out0_bytes; es.out_bytes[1] = out1 ? out1_bytes : 0;
   es.nf = nfields;
+  es.used = 0;
   for (int k = 0; k < nfields; ++k) {
     const uint32_t* f = fields + 5 * k;
     const uint32_t col = f[0], src = f[1], in_shift = f[2], bits = f[3], out_shift = f[4];
@@ -533,6 +592,7 @@ extern "C" int rfx_join_expand_project(rfx_ctx* ctx, const int32_t* lower_bound,
     if (bits == 0 || bits > 128 || in_shift + bits > 8u * es.src_bytes[src] ||
         out_shift + bits > 8u * es.out_bytes[col])
       return fail(RFX_EINVAL, "rfx_join_expand_project: field %d bits out of range", k);
+    es.used |= 1u << src;
     es.f[k] = EmitField{static_cast<uint8_t>(col), static_cast<uint8_t>(src),
                         static_cast<uint8_t>(in_shift), static_cast<uint8_t>(bits),
                         static_cast<uint8_t>(out_shift)};

@@ -5,23 +5,28 @@
 // (Reduce.cu:175-1532: thrust::reduce_by_key with D2H/H2D size copies) and
 // total_double / count / max_double (Reduce.cu:976-1136: thrust::reduce).
 //
-// Floating-point results must not depend on scheduling, so the order of
-// every addition is fixed by the row count alone: the rows are cut into
-// G = 1024 chunks of whole 2048-row tiles (chunk length depends on n only);
-// an upsweep folds each chunk into (heads, open-segment value) in tile
-// order, one block scans the chunk aggregates, and a downsweep replays each
-// chunk with its carry-in and writes one row per run.  The segmented
-// operator is (h1, s1) . (h2, s2) = (h1 + h2, h2 ? s2 : s1 op s2).
+// Reduce-by-key is one pass over the rows.  Tiles of TILE consecutive rows
+// (boundaries at multiples of TILE from row 0) take tickets in order; a
+// thread folds its VT consecutive rows, a fixed shuffle/LDS tree scans the
+// threads with the segmented operator (h1, s1) . (h2, s2) = (h1 + h2, h2 ?
+// s2 : s1 op s2), and the tile publishes its run-head count, the value of
+// its trailing (open) run and the position of its last head.  The run
+// index base comes from a decoupled look-back over the head counts
+// (integers, exact in any order).  A run that began in earlier tiles is
+// finished by the tile where it ends: it walks back to the tile holding the
+// run's head and folds their trailing values right to left, p_s op (p_s+1
+// op (... op p_e-1)), then applies its own part.  Every addition's order is
+// therefore a function of the row count and the data alone, never of
+// scheduling, and each run is written once, with its key, value and row
+// count (end - head + 1).
 #include "rfx_device.h"
 #include "rfx_internal.h"
 
 namespace rfx {
 namespace {
 
-constexpr int kRNT = 256;
-constexpr int kRVT = 8;
-constexpr int kRTile = kRNT * kRVT;
-constexpr uint32_t kRChunks = 1024;  // fixed chunk count: order depends on n only
+constexpr int kBNT = 256;            // threads per reduce-by-key tile
+constexpr uint32_t kRChunks = 1024;  // single reductions: fixed chunk count
 
 struct OpSum {
   template <class V> __device__ static V apply(V a, V b) { return a + b; }
@@ -43,38 +48,23 @@ template <> __device__ uint64_t OpMax::identity<uint64_t>() { return 0ull; }
 
 template <class V>
 struct Seg {
-  uint32_t h;  // heads (run starts) in the span
-  V s;         // value of the span's open (last) run
+  uint32_t h;   // run heads in the span
+  V s;          // value of the span's open (last) run
+  uint64_t lh;  // 1 + row of the span's last head, 0 if none
 };
 
 template <class Op, class V>
-__device__ __forceinline__ Seg<V> seg_combine(Seg<V> a, Seg<V> b) {
-  return {a.h + b.h, b.h ? b.s : Op::apply(a.s, b.s)};
+__device__ __forceinline__ Seg<V> seg_combine(const Seg<V>& a, const Seg<V>& b) {
+  return {a.h + b.h, b.h ? b.s : Op::apply(a.s, b.s), b.lh ? b.lh : a.lh};
 }
 
-// Inclusive segmented scan over the block's threads, Kogge-Stone through
-// LDS (a fixed combination tree).
-template <class Op, class V>
-__device__ __forceinline__ Seg<V> block_seg_scan_incl(Seg<V> x, Seg<V>* sm) {
-  const uint32_t tid = threadIdx.x;
-  sm[tid] = x;
-  __syncthreads();
-  for (uint32_t d = 1; d < kRNT; d <<= 1) {
-    Seg<V> y = x;
-    if (tid >= d) y = seg_combine<Op>(sm[tid - d], x);
-    __syncthreads();
-    x = y;
-    sm[tid] = x;
-    __syncthreads();
-  }
-  return x;
-}
-
-// Values: explicit column (VB > 0) or implicit ones (count).
-template <class V, bool ONES>
-__device__ __forceinline__ V load_val(const V* vals, uint64_t i) {
-  if constexpr (ONES) return V(1);
-  else return vals[i];
+template <class V>
+__device__ __forceinline__ Seg<V> seg_shfl_up(const Seg<V>& x, uint32_t d) {
+  Seg<V> y;
+  y.h = __shfl_up(x.h, d, kWave);
+  y.s = __shfl_up(x.s, d, kWave);
+  y.lh = __shfl_up(x.lh, d, kWave);
+  return y;
 }
 
 struct Sizes {
@@ -85,163 +75,262 @@ struct Sizes {
   uint64_t out_a_mul;
   uint64_t* out_b;        // optional second size (Harmony value bytes)
   uint64_t out_b_mul;
-  uint64_t* n_saved;      // workspace copy of the row count: the size
-                          // variables may be in/out and alias n_ptr
 };
 
 __device__ __forceinline__ uint64_t rows_of(const Sizes& z) {
   uint64_t n = *z.n_ptr / z.n_div;
   return n < z.n_cap ? n : z.n_cap;
 }
-__device__ __forceinline__ uint64_t chunk_tiles(uint64_t n) {
-  uint64_t tiles = (n + kRTile - 1) / kRTile;
-  return (tiles + kRChunks - 1) / kRChunks;
-}
 
-// One tile: loads rows [base, base + kRTile) into LDS and computes each
-// thread's (heads, open value) over its kRVT consecutive rows.
-template <class K, class V, class Op, bool ONES>
-__device__ __forceinline__ Seg<V> load_tile(const K* __restrict__ keys, const V* __restrict__ vals,
-                                            uint64_t n, uint64_t base, K* sk, V* sv,
-                                            uint8_t* head, bool* next_head) {
-  const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < kRTile; i += kRNT) {
-    const uint64_t idx = base + i;
-    if (idx < n) {
-      sk[i] = keys[idx];
-      sv[i] = load_val<V, ONES>(vals, idx);
+template <class K>
+struct RbkTune {
+  static constexpr int VT = sizeof(K) >= 16 ? 8 : 16;
+  static constexpr int TILE = kBNT * VT;
+};
+
+template <class V>
+struct RbkWork {
+  uint64_t* status;    // per tile: look-back word over head counts
+  V* piece;            // per tile: value of its trailing run
+  uint64_t* lasthead;  // per tile: 1 + row of its last head, 0 if none
+  uint32_t* ticket;
+  uint64_t* runs;      // total run count (written by the last tile)
+};
+
+// Rows first .. first + VT - 1 of p into out; rows at or past n are not
+// read.  VEC: 16-byte loads (the column is 16-byte aligned and a thread's
+// rows fill whole 16-byte words).
+template <class T, int VT, bool VEC>
+__device__ __forceinline__ void load_rows(const T* __restrict__ p, uint64_t first, uint64_t n,
+                                          T (&out)[VT]) {
+  if constexpr (VEC && (VT * sizeof(T)) % 16 == 0) {
+    if (first + VT <= n) {
+      const uint4* q = reinterpret_cast<const uint4*>(p + first);
+      constexpr int W = VT * sizeof(T) / 16;
+      uint4 w[W];
+#pragma unroll
+      for (int c = 0; c < W; ++c) w[c] = q[c];
+      __builtin_memcpy(out, w, sizeof(w));
+      return;
     }
   }
-  __syncthreads();
-  for (uint32_t i = tid; i < kRTile; i += kRNT) {
-    const uint64_t idx = base + i;
-    if (idx < n) {
-      bool h;
-      if (idx == 0) h = true;
-      else if (i == 0) h = !eq(keys[idx - 1], sk[0]);
-      else h = !eq(sk[i - 1], sk[i]);
-      head[i] = h;
-    }
-  }
-  if (tid == 0) {
-    const uint64_t nx = base + kRTile;
-    *next_head = nx >= n ? true : !eq(keys[nx - 1], keys[nx]);
-  }
-  __syncthreads();
-  Seg<V> t{0, Op::template identity<V>()};
-  for (int j = 0; j < kRVT; ++j) {
-    const uint32_t i = tid * kRVT + j;
-    if (base + i >= n) break;
-    if (head[i]) t = {t.h + 1, sv[i]};
-    else t.s = Op::apply(t.s, sv[i]);
-  }
-  return t;
+#pragma unroll
+  for (int j = 0; j < VT; ++j)
+    if (first + j < n) out[j] = p[first + j];
 }
 
-template <class K, class V, class Op, bool ONES>
-__global__ __launch_bounds__(kRNT) void k_rbk_up(const K* __restrict__ keys,
-                                                 const V* __restrict__ vals, Sizes z,
-                                                 Seg<V>* __restrict__ chunk_agg) {
-  __shared__ K sk[kRTile];
-  __shared__ V sv[kRTile];
-  __shared__ uint8_t head[kRTile];
-  __shared__ Seg<V> sm[kRNT];
-  __shared__ bool next_head;
+template <class K>
+__device__ __forceinline__ K prev_of_tile(const K* keys, uint64_t base) {
+  return keys[base - 1];
+}
+
+// MODE bit 0: values (Op over vals); bit 1: row counts.
+template <class K, class V, class Op, int MODE, bool VEC>
+__global__ __launch_bounds__(kBNT) void k_rbk(const K* __restrict__ keys,
+                                              const V* __restrict__ vals, Sizes z, RbkWork<V> w,
+                                              K* __restrict__ out_keys, V* __restrict__ out_vals,
+                                              uint32_t* __restrict__ out_counts) {
+  constexpr int VT = RbkTune<K>::VT;
+  constexpr int TILE = RbkTune<K>::TILE;
+  constexpr int NW = kBNT / kWave;
+  constexpr bool HAS_V = MODE & 1;
+  constexpr bool HAS_C = MODE & 2;
+  __shared__ K s_first[kBNT];
+  __shared__ K s_last[kBNT];
+  __shared__ Seg<V> s_wave[NW];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_excl;
+  __shared__ V s_carry;
+  __shared__ uint64_t s_carry_head;
+
   const uint64_t n = rows_of(z);
-  const uint64_t ct = chunk_tiles(n);
-  const uint64_t c = blockIdx.x;
-  if (n == 0 || c * ct * kRTile >= n) return;
-  Seg<V> agg{0, Op::template identity<V>()};
-  for (uint64_t t = 0; t < ct; ++t) {
-    const uint64_t base = (c * ct + t) * kRTile;
-    if (base >= n) break;
-    Seg<V> mine = load_tile<K, V, Op, ONES>(keys, vals, n, base, sk, sv, head, &next_head);
-    Seg<V> incl = block_seg_scan_incl<Op>(mine, sm);
-    if (threadIdx.x == kRNT - 1) agg = seg_combine<Op>(agg, incl);
-    __syncthreads();
-  }
-  if (threadIdx.x == kRNT - 1) chunk_agg[c] = agg;
-}
-
-// One block: exclusive segmented scan of the chunk aggregates; writes the
-// run count.
-template <class V, class Op>
-__global__ __launch_bounds__(kRChunks) void k_rbk_spine(Seg<V>* __restrict__ chunk_agg, Sizes z) {
-  __shared__ Seg<V> sm[kRChunks];
-  __shared__ uint32_t tot;
-  const uint64_t n = rows_of(z);
-  const uint64_t ct = chunk_tiles(n);
-  const uint64_t nch = n ? (n + ct * kRTile - 1) / (ct * kRTile) : 0;
-  const uint32_t tid = threadIdx.x;
-  Seg<V> x = tid < nch ? chunk_agg[tid] : Seg<V>{0, Op::template identity<V>()};
-  sm[tid] = x;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid / kWave;
+  if (tid == 0) s_tile = atomicAdd(w.ticket, 1u);
   __syncthreads();
-  for (uint32_t d = 1; d < kRChunks; d <<= 1) {
-    Seg<V> y = x;
-    if (tid >= d) y = seg_combine<Op>(sm[tid - d], x);
-    __syncthreads();
-    x = y;
-    sm[tid] = x;
-    __syncthreads();
-  }
-  Seg<V> excl = tid ? sm[tid - 1] : Seg<V>{0, Op::template identity<V>()};
-  if (tid == kRChunks - 1) tot = x.h;
-  __syncthreads();
-  if (tid < nch) chunk_agg[tid] = excl;
-  if (tid == 0) {
-    *z.n_saved = n;  // every thread has read *n_ptr before the barriers above
-    *z.out_a = static_cast<uint64_t>(tot) * z.out_a_mul;
-    if (z.out_b) *z.out_b = static_cast<uint64_t>(tot) * z.out_b_mul;
-  }
-}
+  const uint32_t tile = s_tile;
+  const uint64_t base = static_cast<uint64_t>(tile) * TILE;
+  if (base >= n) return;
+  const uint64_t first = base + static_cast<uint64_t>(tid) * VT;
 
-template <class K, class V, class Op, bool ONES>
-__global__ __launch_bounds__(kRNT) void k_rbk_down(const K* __restrict__ keys,
-                                                   const V* __restrict__ vals, Sizes z,
-                                                   const Seg<V>* __restrict__ chunk_carry,
-                                                   K* __restrict__ out_keys,
-                                                   V* __restrict__ out_vals) {
-  __shared__ K sk[kRTile];
-  __shared__ V sv[kRTile];
-  __shared__ uint8_t head[kRTile];
-  __shared__ Seg<V> sm[kRNT];
-  __shared__ bool next_head;
-  const uint64_t n = *z.n_saved;
-  const uint64_t ct = chunk_tiles(n);
-  const uint64_t c = blockIdx.x;
-  if (n == 0 || c * ct * kRTile >= n) return;
-  const uint32_t tid = threadIdx.x;
-  Seg<V> carry = chunk_carry[c];  // heads before the chunk, open value
-  for (uint64_t t = 0; t < ct; ++t) {
-    const uint64_t base = (c * ct + t) * kRTile;
-    if (base >= n) break;
-    Seg<V> mine = load_tile<K, V, Op, ONES>(keys, vals, n, base, sk, sv, head, &next_head);
-    Seg<V> incl = block_seg_scan_incl<Op>(mine, sm);
-    Seg<V> before = tid ? sm[tid - 1] : Seg<V>{0, Op::template identity<V>()};
-    Seg<V> in = seg_combine<Op>(carry, before);  // state entering this thread
-    // Replay the thread's rows.
-    uint64_t run = static_cast<uint64_t>(in.h) - 1;  // index of the open run
-    V acc = in.s;
-    for (int j = 0; j < kRVT; ++j) {
-      const uint32_t i = tid * kRVT + j;
-      const uint64_t idx = base + i;
-      if (idx >= n) break;
-      if (head[i]) {
-        ++run;
-        acc = sv[i];
+  K k[VT];
+  V v[VT];
+  load_rows<K, VT, VEC>(keys, first, n, k);
+  if constexpr (HAS_V) load_rows<V, VT, VEC>(vals, first, n, v);
+  s_first[tid] = k[0];
+  s_last[tid] = k[VT - 1];
+  __syncthreads();
+  K prev{}, next{};
+  if (tid) prev = s_last[tid - 1];
+  else if (base) prev = keys[base - 1];
+  if (tid + 1 < kBNT) next = s_first[tid + 1];
+  else if (base + TILE < n) next = keys[base + TILE];
+
+  // Heads and run ends of the thread's rows; the thread's aggregate.
+  uint32_t hmask = 0, emask = 0;
+  Seg<V> mine{0, V{}, 0};
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < VT; ++j) {
+    const uint64_t row = first + j;
+    if (row < n) {
+      const K pk = j ? k[j - 1] : prev;
+      const K nk = j + 1 < VT ? k[j + 1] : next;
+      const bool head = row == 0 || !eq(pk, k[j]);
+      const bool end = row + 1 >= n || !eq(k[j], nk);
+      hmask |= static_cast<uint32_t>(head) << j;
+      emask |= static_cast<uint32_t>(end) << j;
+      V x{};
+      if constexpr (HAS_V) x = v[j];
+      if (head) {
+        ++mine.h;
+        mine.s = x;
+        mine.lh = row + 1;
       } else {
-        acc = Op::apply(acc, sv[i]);
+        mine.s = any ? Op::apply(mine.s, x) : x;
       }
-      const bool last = (i + 1 < kRTile) ? (idx + 1 >= n || head[i + 1]) : next_head;
-      if (last) {
-        out_keys[run] = sk[i];
-        out_vals[run] = acc;
+      any = true;
+    }
+  }
+
+  // Block scan of the thread aggregates (threads past the last row hold
+  // h = 0 and never feed a result).
+  Seg<V> incl = mine;
+#pragma unroll
+  for (uint32_t d = 1; d < kWave; d <<= 1) {
+    const Seg<V> y = seg_shfl_up(incl, d);
+    if (lane >= d) incl = seg_combine<Op>(y, incl);
+  }
+  if (lane == kWave - 1) s_wave[wv] = incl;
+  Seg<V> excl_w = seg_shfl_up(incl, 1);  // exclusive within the wave (lane 0: none)
+  __syncthreads();
+  Seg<V> before{0, V{}, 0};
+  bool have_before = false;
+  for (uint32_t i = 0; i < wv; ++i) {
+    before = have_before ? seg_combine<Op>(before, s_wave[i]) : s_wave[i];
+    have_before = true;
+  }
+  if (lane) {
+    before = have_before ? seg_combine<Op>(before, excl_w) : excl_w;
+    have_before = true;
+  }
+  Seg<V> total = s_wave[0];
+  for (int i = 1; i < NW; ++i) total = seg_combine<Op>(total, s_wave[i]);
+  // Does the tile's first row continue a run from an earlier tile?
+  const bool continues = base != 0 && eq(prev_of_tile(keys, base), s_first[0]);
+
+  // Publish, then the look-back (wave 0) and the carry walk (wave 1).
+  if (tid == 0) {
+    w.piece[tile] = total.s;
+    w.lasthead[tile] = total.lh;
+    __threadfence();
+    status_store64(w.status + tile, (tile == 0 ? kFlagIncl64 : kFlagAgg64) | total.h);
+  }
+  if (wv == 0) {
+    uint64_t ex = 0;
+    if (tile) {
+      ex = wave_lookback64(w.status, tile);
+      if (tid == 0) status_store64(w.status + tile, kFlagIncl64 | (ex + total.h));
+    }
+    if (tid == 0) {
+      s_excl = ex;
+      if (base + TILE >= n) *w.runs = ex + total.h;
+    }
+  } else if (wv == 1 && continues) {
+    // Walk back to the tile holding the open run's head, folding the
+    // trailing values right to left in walk order.
+    V acc{};
+    bool have = false;
+    uint64_t head_row = 0;
+    int64_t t0 = static_cast<int64_t>(tile) - 1;
+    bool done = false;
+    while (!done) {
+      const int64_t kt = t0 - static_cast<int64_t>(lane);
+      bool pub = kt < 0;  // tile 0 always holds a head, so lanes past it are never folded
+      while (true) {
+        if (!pub) pub = (status_load64(w.status + kt) >> 62) != 0;
+        const uint64_t ready = ballot(pub);
+        const uint32_t prefix = ready == ~0ull ? 64u : static_cast<uint32_t>(__builtin_ctzll(~ready));
+        if (prefix == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        __threadfence();
+        const bool in = lane < prefix && kt >= 0;
+        const uint64_t lh = in ? w.lasthead[kt] : 0;
+        const V pc = in ? w.piece[kt] : V{};
+        const uint64_t hm = ballot(in && lh != 0);
+        if (!hm && prefix < 64) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        const uint32_t stop = hm ? static_cast<uint32_t>(__builtin_ctzll(hm)) : 63u;
+        for (uint32_t j = 0; j <= stop; ++j) {
+          const V pj = __shfl(pc, j, kWave);
+          acc = have ? Op::apply(pj, acc) : pj;
+          have = true;
+        }
+        if (hm) {
+          head_row = __shfl(lh, stop, kWave) - 1;
+          done = true;
+        } else {
+          t0 -= kWave;
+        }
+        break;
       }
     }
-    const Seg<V> tile_all = sm[kRNT - 1];
-    __syncthreads();
-    carry = seg_combine<Op>(carry, tile_all);
-    (void)incl;
+    if (lane == 0) {
+      s_carry = acc;
+      s_carry_head = head_row;
+    }
+  }
+  __syncthreads();
+
+  // Replay: write every run that ends in the thread's rows.
+  if (!any) return;
+  uint64_t run = s_excl + before.h - 1;  // index of the run open on entry
+  V val{};
+  uint64_t hrow = 0;
+  bool open = false;
+  if (have_before && before.h) {  // the open run began earlier in this tile
+    val = before.s;
+    hrow = before.lh - 1;
+    open = true;
+  } else if (continues) {  // it began in an earlier tile
+    val = have_before ? Op::apply(s_carry, before.s) : s_carry;
+    hrow = s_carry_head;
+    open = true;
+  }
+#pragma unroll
+  for (int j = 0; j < VT; ++j) {
+    const uint64_t row = first + j;
+    if (row < n) {
+      V x{};
+      if constexpr (HAS_V) x = v[j];
+      if ((hmask >> j) & 1) {
+        ++run;
+        val = x;
+        hrow = row;
+        open = true;
+      } else {
+        val = open ? Op::apply(val, x) : x;
+        open = true;
+      }
+      if ((emask >> j) & 1) {
+        out_keys[run] = k[j];
+        if constexpr (HAS_V) out_vals[run] = val;
+        if constexpr (HAS_C) out_counts[run] = static_cast<uint32_t>(row - hrow + 1);
+      }
+    }
+  }
+}
+
+// Sizes after the pass: the size variables may alias the row count, so
+// they are written only once every tile has read it.
+__global__ void k_rbk_sizes(Sizes z, const uint64_t* __restrict__ runs, uint64_t n_cap) {
+  if (threadIdx.x == 0) {
+    const uint64_t r = n_cap ? *runs : 0;
+    *z.out_a = r * z.out_a_mul;
+    if (z.out_b) *z.out_b = r * z.out_b_mul;
   }
 }
 
@@ -278,49 +367,62 @@ __global__ __launch_bounds__(kRChunks) void k_reduce_final(const V* __restrict__
   if (threadIdx.x == 0) *out = sm[0];
 }
 
-template <class K, class V, class Op, bool ONES>
-int rbk_impl(rfx_ctx* ctx, void* out_keys, void* out_vals, const void* keys, const void* vals,
-             const Sizes& z) {
+template <class K, class V, class Op, int MODE>
+int rbk_impl(rfx_ctx* ctx, void* out_keys, void* out_vals, uint32_t* out_counts,
+             const void* keys, const void* vals, const Sizes& z) {
   if (z.n_cap == 0) {
     RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(z.out_a), 2, 0));
     if (z.out_b) RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(z.out_b), 2, 0));
     return RFX_OK;
   }
-  Carve w0(nullptr);
-  w0.take<Seg<V>>(kRChunks);
-  w0.take<uint64_t>(1);
-  RFX_TRY(ensure_workspace(ctx, w0.off));
-  Carve w(ctx->ws);
-  Seg<V>* agg = w.take<Seg<V>>(kRChunks);
-  Sizes zz = z;
-  zz.n_saved = w.take<uint64_t>(1);
-  const uint64_t tiles_cap = (z.n_cap + kRTile - 1) / kRTile;
-  const uint32_t grid = static_cast<uint32_t>(tiles_cap < kRChunks ? tiles_cap : kRChunks);
+  constexpr int TILE = RbkTune<K>::TILE;
+  const uint64_t ntiles = (z.n_cap + TILE - 1) / TILE;
+  Carve c(nullptr);
+  c.take<uint64_t>(ntiles);
+  c.take<uint32_t>(1);
+  c.take<uint64_t>(1);
+  c.take<V>(ntiles);
+  c.take<uint64_t>(ntiles);
+  RFX_TRY(ensure_workspace(ctx, c.off));
+  Carve cw(ctx->ws);
+  RbkWork<V> w;
+  w.status = cw.take<uint64_t>(ntiles);
+  w.ticket = cw.take<uint32_t>(1);
+  w.runs = cw.take<uint64_t>(1);
+  w.piece = cw.take<V>(ntiles);
+  w.lasthead = cw.take<uint64_t>(ntiles);
+  // status, ticket and runs are contiguous after alignment; zero them.
+  RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(w.status),
+                   (reinterpret_cast<char*>(w.runs + 1) - reinterpret_cast<char*>(w.status)) / 4,
+                   0));
+  const bool vec = (reinterpret_cast<uintptr_t>(keys) % 16 == 0) &&
+                   (!(MODE & 1) || reinterpret_cast<uintptr_t>(vals) % 16 == 0);
   {
-    RFX_PROF(ctx, "rbk.up");
-    k_rbk_up<K, V, Op, ONES><<<grid, kRNT, 0, ctx->stream>>>(
-        static_cast<const K*>(keys), static_cast<const V*>(vals), zz, agg);
+    RFX_PROF(ctx, "rbk");
+    if (vec)
+      k_rbk<K, V, Op, MODE, true><<<static_cast<uint32_t>(ntiles), kBNT, 0, ctx->stream>>>(
+          static_cast<const K*>(keys), static_cast<const V*>(vals), z, w,
+          static_cast<K*>(out_keys), static_cast<V*>(out_vals), out_counts);
+    else
+      k_rbk<K, V, Op, MODE, false><<<static_cast<uint32_t>(ntiles), kBNT, 0, ctx->stream>>>(
+          static_cast<const K*>(keys), static_cast<const V*>(vals), z, w,
+          static_cast<K*>(out_keys), static_cast<V*>(out_vals), out_counts);
     RFX_LAUNCHED();
   }
-  k_rbk_spine<V, Op><<<1, kRChunks, 0, ctx->stream>>>(agg, zz);
-  RFX_LAUNCHED();
-  RFX_PROF(ctx, "rbk.down");
-  k_rbk_down<K, V, Op, ONES><<<grid, kRNT, 0, ctx->stream>>>(
-      static_cast<const K*>(keys), static_cast<const V*>(vals), zz, agg, static_cast<K*>(out_keys),
-      static_cast<V*>(out_vals));
+  k_rbk_sizes<<<1, 64, 0, ctx->stream>>>(z, w.runs, z.n_cap);
   RFX_LAUNCHED();
   return RFX_OK;
 }
 
-template <class V, class Op, bool ONES>
-int rbk_keys(rfx_ctx* ctx, int kb, void* ok, void* ov, const void* k, const void* v,
-             const Sizes& z) {
+template <class V, class Op, int MODE>
+int rbk_keys(rfx_ctx* ctx, int kb, void* ok, void* ov, uint32_t* oc, const void* k,
+             const void* v, const Sizes& z) {
   switch (kb) {
-    case 1: return rbk_impl<uint8_t, V, Op, ONES>(ctx, ok, ov, k, v, z);
-    case 2: return rbk_impl<uint16_t, V, Op, ONES>(ctx, ok, ov, k, v, z);
-    case 4: return rbk_impl<uint32_t, V, Op, ONES>(ctx, ok, ov, k, v, z);
-    case 8: return rbk_impl<uint64_t, V, Op, ONES>(ctx, ok, ov, k, v, z);
-    case 16: return rbk_impl<u128, V, Op, ONES>(ctx, ok, ov, k, v, z);
+    case 1: return rbk_impl<uint8_t, V, Op, MODE>(ctx, ok, ov, oc, k, v, z);
+    case 2: return rbk_impl<uint16_t, V, Op, MODE>(ctx, ok, ov, oc, k, v, z);
+    case 4: return rbk_impl<uint32_t, V, Op, MODE>(ctx, ok, ov, oc, k, v, z);
+    case 8: return rbk_impl<uint64_t, V, Op, MODE>(ctx, ok, ov, oc, k, v, z);
+    case 16: return rbk_impl<u128, V, Op, MODE>(ctx, ok, ov, oc, k, v, z);
   }
   return fail(RFX_EINVAL, "reduce_by_key: key width %d", kb);
 }
@@ -343,28 +445,41 @@ int single_impl(rfx_ctx* ctx, V* result, const V* v, uint64_t n) {
 }  // namespace
 
 // Shared with harmony.cpp: reduce-by-key with byte-size variables.
+// val_kind: 0 f64 (op), 1 u32 sum, 2 count (out_vals receives u32 counts),
+// 3 f64 (op) with counts in out_counts as well.
 int reduce_by_key_sized(rfx_ctx* ctx, int op, int val_kind, int kb, void* out_keys,
                         void* out_vals, const void* keys, const void* vals,
                         const uint64_t* n_ptr, uint64_t n_div, uint64_t n_cap, uint64_t* out_a,
-                        uint64_t out_a_mul, uint64_t* out_b, uint64_t out_b_mul) {
-  Sizes z{n_ptr, n_div, n_cap, out_a, out_a_mul, out_b, out_b_mul, nullptr};
+                        uint64_t out_a_mul, uint64_t* out_b, uint64_t out_b_mul,
+                        uint32_t* out_counts) {
+  Sizes z{n_ptr, n_div, n_cap, out_a, out_a_mul, out_b, out_b_mul};
   if (!ctx || !n_ptr || !out_a) return fail(RFX_EINVAL, "reduce_by_key: null argument");
-  if (n_cap && (!keys || !out_keys || !out_vals || (val_kind != 2 && !vals)))
+  if (n_cap && (!keys || !out_keys || !out_vals || (val_kind != 2 && !vals) ||
+                (val_kind == 3 && !out_counts)))
     return fail(RFX_EINVAL, "reduce_by_key: null column");
   if (n_cap >= (1ull << 31)) return fail(RFX_ERANGE, "reduce_by_key: too many rows");
-  if (val_kind == 0) {  // f64
+  if (val_kind == 0 || val_kind == 3) {  // f64
+    if (val_kind == 3) {
+      switch (op) {
+        case RFX_AGG_SUM: return rbk_keys<double, OpSum, 3>(ctx, kb, out_keys, out_vals, out_counts, keys, vals, z);
+        case RFX_AGG_MIN: return rbk_keys<double, OpMin, 3>(ctx, kb, out_keys, out_vals, out_counts, keys, vals, z);
+        case RFX_AGG_MAX: return rbk_keys<double, OpMax, 3>(ctx, kb, out_keys, out_vals, out_counts, keys, vals, z);
+      }
+      return fail(RFX_EINVAL, "reduce_by_key: op %d", op);
+    }
     switch (op) {
-      case RFX_AGG_SUM: return rbk_keys<double, OpSum, false>(ctx, kb, out_keys, out_vals, keys, vals, z);
-      case RFX_AGG_MIN: return rbk_keys<double, OpMin, false>(ctx, kb, out_keys, out_vals, keys, vals, z);
-      case RFX_AGG_MAX: return rbk_keys<double, OpMax, false>(ctx, kb, out_keys, out_vals, keys, vals, z);
+      case RFX_AGG_SUM: return rbk_keys<double, OpSum, 1>(ctx, kb, out_keys, out_vals, nullptr, keys, vals, z);
+      case RFX_AGG_MIN: return rbk_keys<double, OpMin, 1>(ctx, kb, out_keys, out_vals, nullptr, keys, vals, z);
+      case RFX_AGG_MAX: return rbk_keys<double, OpMax, 1>(ctx, kb, out_keys, out_vals, nullptr, keys, vals, z);
     }
     return fail(RFX_EINVAL, "reduce_by_key: op %d", op);
   }
   if (val_kind == 1) {  // u32 sum
     if (op != RFX_AGG_SUM) return fail(RFX_ENOTSUP, "reduce_by_key: u32 supports sum only");
-    return rbk_keys<uint32_t, OpSum, false>(ctx, kb, out_keys, out_vals, keys, vals, z);
+    return rbk_keys<uint32_t, OpSum, 1>(ctx, kb, out_keys, out_vals, nullptr, keys, vals, z);
   }
-  return rbk_keys<uint32_t, OpSum, true>(ctx, kb, out_keys, out_vals, keys, nullptr, z);
+  return rbk_keys<uint32_t, OpSum, 2>(ctx, kb, out_keys, nullptr, static_cast<uint32_t*>(out_vals),
+                                      keys, nullptr, z);
 }
 
 int single_reduce(rfx_ctx* ctx, int kind, void* result, const void* v, uint64_t n) {
@@ -385,21 +500,30 @@ extern "C" int rfx_reduce_by_key_f64(rfx_ctx* ctx, void* out_keys, double* out_v
                                      uint64_t* n_out, const void* keys, const double* vals,
                                      const uint64_t* n, uint64_t n_cap, int key_bytes, int op) {
   return reduce_by_key_sized(ctx, op, 0, key_bytes, out_keys, out_vals, keys, vals, n, 1, n_cap,
-                             n_out, 1, nullptr, 0);
+                             n_out, 1, nullptr, 0, nullptr);
+}
+
+extern "C" int rfx_reduce_count_by_key_f64(rfx_ctx* ctx, void* out_keys, double* out_vals,
+                                           uint32_t* out_counts, uint64_t* n_out,
+                                           const void* keys, const double* vals,
+                                           const uint64_t* n, uint64_t n_cap, int key_bytes,
+                                           int op) {
+  return reduce_by_key_sized(ctx, op, 3, key_bytes, out_keys, out_vals, keys, vals, n, 1, n_cap,
+                             n_out, 1, nullptr, 0, out_counts);
 }
 
 extern "C" int rfx_reduce_by_key_u32(rfx_ctx* ctx, void* out_keys, uint32_t* out_vals,
                                      uint64_t* n_out, const void* keys, const uint32_t* vals,
                                      const uint64_t* n, uint64_t n_cap, int key_bytes) {
   return reduce_by_key_sized(ctx, RFX_AGG_SUM, 1, key_bytes, out_keys, out_vals, keys, vals, n, 1,
-                             n_cap, n_out, 1, nullptr, 0);
+                             n_cap, n_out, 1, nullptr, 0, nullptr);
 }
 
 extern "C" int rfx_count_by_key(rfx_ctx* ctx, void* out_keys, uint32_t* out_counts,
                                 uint64_t* n_out, const void* keys, const uint64_t* n,
                                 uint64_t n_cap, int key_bytes) {
   return reduce_by_key_sized(ctx, RFX_AGG_SUM, 2, key_bytes, out_keys, out_counts, keys, nullptr,
-                             n, 1, n_cap, n_out, 1, nullptr, 0);
+                             n, 1, n_cap, n_out, 1, nullptr, 0, nullptr);
 }
 
 extern "C" int rfx_total_f64(rfx_ctx* ctx, double* result, const double* v, uint64_t n) {

@@ -107,11 +107,13 @@ inline int bytes_of_bits(uint64_t bits) {
 
 // Reduce-by-key with size variables: rows = *n_ptr / n_div (at most
 // n_cap); writes runs * out_a_mul to *out_a and runs * out_b_mul to *out_b.
-// val_kind: 0 f64 (op = RFX_AGG_*), 1 u32 sum, 2 count (implicit ones).
+// val_kind: 0 f64 (op = RFX_AGG_*), 1 u32 sum, 2 count (u32 counts to
+// out_vals), 3 f64 with the row count of each run also to out_counts.
 int reduce_by_key_sized(rfx_ctx* ctx, int op, int val_kind, int kb, void* out_keys,
                         void* out_vals, const void* keys, const void* vals,
                         const uint64_t* n_ptr, uint64_t n_div, uint64_t n_cap, uint64_t* out_a,
-                        uint64_t out_a_mul, uint64_t* out_b, uint64_t out_b_mul);
+                        uint64_t out_a_mul, uint64_t* out_b, uint64_t out_b_mul,
+                        uint32_t* out_counts);
 // kind 0: f64 sum, 1: u32 sum, 2: max of u64 bit patterns.
 int single_reduce(rfx_ctx* ctx, int kind, void* result, const void* v, uint64_t n);
 

@@ -358,15 +358,15 @@ class Plan {
     RFX_TRY(step(rfx_join_expand_project(ctx_, lb, lc, nf, tot, t, fk, 4, m, 8, dg, 4, key2, 4,
                                          val, 8, 2, fields)));
     RFX_TRY(step(rfx_sort_pairs(ctx_, key2, val, t, 4, static_cast<int>(gbits), 8)));
-    RFX_TRY(step(rfx_reduce_by_key_f64(ctx_, ok, osum, nout, key2, val, tot, t, 4, RFX_AGG_SUM)));
-    if (plain_) {
+    if (plain_) {  // Harmony: set_reduce, then set_count, sizes read back
       uint64_t g = 0;
+      RFX_TRY(step(rfx_reduce_by_key_f64(ctx_, ok, osum, nout, key2, val, tot, t, 4, RFX_AGG_SUM)));
       RFX_TRY(read(nout, &g));
-    }
-    RFX_TRY(step(rfx_count_by_key(ctx_, ok2, ocnt, nout2, key2, tot, t, 4)));
-    if (plain_) {
-      uint64_t g = 0;
+      RFX_TRY(step(rfx_count_by_key(ctx_, ok2, ocnt, nout2, key2, tot, t, 4)));
       RFX_TRY(read(nout2, &g));
+    } else {  // both aggregates in one pass (the same bits)
+      RFX_TRY(step(rfx_reduce_count_by_key_f64(ctx_, ok, osum, ocnt, nout, key2, val, tot, t, 4,
+                                               RFX_AGG_SUM)));
     }
     RFX_TRY(finish(nout, groups));
     RFX_TRY(digest_col<4>(0, ok, groups));

@@ -95,9 +95,20 @@ __global__ __launch_bounds__(256) void k_radix_hist(const typename Elem<KB>::T* 
   __syncthreads();
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
   const uint64_t gtid = blockIdx.x * 256ull + threadIdx.x;
+  // A digit shared by every active lane of the wave (sorted or
+  // low-cardinality keys) is added once by the first lane instead of 64
+  // times to the same LDS word.
   auto count = [&](K k) {
-    for (int p = 0; p < passes; ++p)
-      atomicAdd(&lh[p * kBins + digit_of(k, 8u * p, pass_mask(p, key_bits))], 1u);
+    const uint64_t active = ballot(true);
+    for (int p = 0; p < passes; ++p) {
+      const uint32_t d = digit_of(k, 8u * p, pass_mask(p, key_bits));
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+      if (ballot(d != d0) == 0) {
+        if (mbcnt(active) == 0) atomicAdd(&lh[p * kBins + d], static_cast<uint32_t>(__popcll(active)));
+      } else {
+        atomicAdd(&lh[p * kBins + d], 1u);
+      }
+    }
   };
   uint64_t done = 0;
   if constexpr (PER > 1) {

@@ -250,6 +250,12 @@ class Context:
             self.handle, out_keys.ptr, out_counts.ptr, n_out.ptr, keys.ptr, n.ptr,
             n_cap, key_bytes))
 
+    def reduce_count_by_key_f64(self, out_keys, out_vals, out_counts, n_out, keys, vals, n,
+                                n_cap, key_bytes, op=0) -> None:
+        _check("rfx_reduce_count_by_key_f64", lib().rfx_reduce_count_by_key_f64(
+            self.handle, out_keys.ptr, out_vals.ptr, out_counts.ptr, n_out.ptr, keys.ptr,
+            vals.ptr, n.ptr, n_cap, key_bytes, op))
+
     def total_f64(self, result, v, n) -> None:
         _check("rfx_total_f64", lib().rfx_total_f64(self.handle, result.ptr, v.ptr, n))
 

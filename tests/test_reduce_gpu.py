@@ -115,6 +115,33 @@ def test_deterministic_bits(rf, ctx, orc):
     assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
+@pytest.mark.parametrize("kb,groups,n", [(4, 20_000, 300_001), (8, 3, 2_000_003), (16, 1, 70_000)])
+def test_fused_sum_and_count(rf, ctx, orc, kb, groups, n):
+    """One pass for SUM and COUNT(*): the same bits as the two separate
+    reductions, counts exact, sums within 1e-12 of the oracle."""
+    keys = sorted_keys(orc, kb, n, 40 + kb, groups)
+    vals = datagen.uniform01(41 + kb, n) * 1e4
+    K = rf.DeviceBuffer.from_numpy(keys)
+    V = rf.DeviceBuffer.from_numpy(vals)
+    OK, OV, OC = rf.DeviceBuffer(keys.nbytes), rf.DeviceBuffer(8 * n), rf.DeviceBuffer(4 * n)
+    nout, ncell = rf.u64_cell(5), rf.u64_cell(n)
+    # Capacity above the row count: only *n rows count.
+    ctx.reduce_count_by_key_f64(OK, OV, OC, nout, K, V, ncell, n + 12345, kb, 0)
+    ctx.synchronize()
+    g = rf.read_u64(nout)
+    fk = OK.download(keys.dtype, g * kb // keys.itemsize)
+    fv, fc = OV.download(np.float64, g), OC.download(np.uint32, g)
+    sk, sv = gpu_rbk(rf, ctx, keys, vals, kb)
+    ck, cc = gpu_rbk(rf, ctx, keys, None, kb, kind="count")
+    assert np.array_equal(fk, sk) and np.array_equal(fk, ck)
+    assert np.array_equal(fv.view(np.uint64), sv.view(np.uint64))
+    assert np.array_equal(fc, cc)
+    ok, ov = orc.reduce_by_key_f64(keys, vals, kb, 0)
+    _, oc = orc.reduce_by_key_u32(keys, np.ones(n, dtype=np.uint32), kb)
+    assert np.array_equal(fk, ok) and np.array_equal(fc, oc)
+    np.testing.assert_allclose(fv, ov, rtol=REL, atol=0)
+
+
 def test_single_reductions(rf, ctx, orc):
     n = 3_000_001
     v = datagen.uniform01(31, n) * 1e6
