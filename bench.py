@@ -41,13 +41,52 @@ def median(xs):
     return float(np.median(np.array(xs)))
 
 
+COPY_GBPS = None  # measured device-to-device copy rate (read + write bytes)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sort_onesweep_2p28_u32_traffic.txt")
+
+
 def roofline(bytes_per_launch, launches, total_ms, traffic=None):
     avg_ms = total_ms / max(launches, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "launch_ms": round(avg_ms, 4), "launches": launches,
-            "bytes_per_launch": int(bytes_per_launch)}
+    r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+         "launch_ms": round(avg_ms, 4), "launches": launches,
+         "bytes_per_launch": int(bytes_per_launch)}
+    if COPY_GBPS:
+        r["frac_of_copy"] = round(achieved / COPY_GBPS, 4)
+    return r
+
+
+def copy_bandwidth(rf, ctx, nbytes=1 << 30, reps=5):
+    """Device copy of 1 GiB (the reference kernel of known size for the
+    traffic counters): bytes read + written per second."""
+    a, b = rf.DeviceBuffer(nbytes), rf.DeviceBuffer(nbytes)
+    ctx.copy(b, a, nbytes)
+    ctx.synchronize()
+    t = rf.Timer(ctx)
+    times = []
+    for _ in range(reps):
+        t.begin()
+        ctx.copy(b, a, nbytes)
+        times.append(t.end())
+    t.close()
+    a.close()
+    b.close()
+    return 2 * nbytes / (median(times) * 1e-3) / 1e9
+
+
+def measured_traffic():
+    """HBM bytes per onesweep launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on the 1 GiB
+    copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 # ---------------------------------------------------------------------------
@@ -88,7 +127,7 @@ def bench_sort_u32(rf, ctx, steps, warmup):
     for b in keys + vals:
         b.close()
     # Each onesweep launch reads and writes every (key, value): 16 B/row.
-    rl = roofline(16 * n, launches, sweep_ms)
+    rl = roofline(16 * n, launches, sweep_ms, measured_traffic())
     return {"n": n, "wall_s": wall, "event_ms": ev_ms, "ms_per_step": 1e3 * wall / steps,
             "rows_per_s": n * steps / wall, "roofline": rl, "verified_sorted": ok,
             "hist_ms": hist[1] / max(hist[0], 1),
@@ -234,16 +273,21 @@ def bench_agg(rf, ctx, iters=5):
     for _ in range(iters + 1):
         t = rf.Timer(ctx)
         t.begin()
-        ctx.reduce_count_by_key_f64(ok, ov, oc, nout, keys, vals, ncell, n, 4, 0)
+        ctx.reduce_count_by_key_f64(ok, ov, oc, nout, keys, vals, ncell, n, 4, 0, out_cap=groups)
         times.append(t.end())
         t.close()
-    launches, ms = ctx.profile_read("rbk")
+    # Two passes (upsweep, downsweep) per call; the roofline charges the
+    # algorithmic bytes (each row read once, each group written once) to
+    # their combined time.
+    lu, mu = ctx.profile_read("rbk.up")
+    ld, md = ctx.profile_read("rbk.down")
     ctx.profile(False)
     g = rf.read_u64(nout)
     cnt = oc.download(np.uint32, g)
     res = {"n": n, "groups": g, "ms": median(times[1:]), "rows_per_s": n / (median(times[1:]) * 1e-3),
            "counts_sum_to_n": bool(int(cnt.sum(dtype=np.uint64)) == n),
-           "roofline": roofline(12 * n + 16 * g, launches, ms)}
+           "roofline": roofline(12 * n + 16 * g, lu, mu + md),
+           "upsweep_ms": mu / max(lu, 1), "downsweep_ms": md / max(ld, 1)}
     for b in (keys, vals, ident, row_off, dim_off, ok, ov, oc, nout, ncell):
         b.close()
     return res
@@ -314,7 +358,8 @@ def bench_batched(rf, ctx, reps=5):
     """512 join + group-aggregate instances (fact 10^U[3,7] rows, Zipf(1.1)
     foreign keys into dimensions of 10^U[2,5] rows): the plain per-operator
     form (one instance and one operator launch at a time, sizes read back
-    as Harmony does) against the same plan batched into one HIP graph."""
+    as Harmony does) against the same plan batched into one HIP graph, its
+    tile shapes chosen first by the autotuner (not timed)."""
     from redfox_amd import batched
 
     w = batched.Workload(ctx, n_inst=512)
@@ -326,6 +371,7 @@ def bench_batched(rf, ctx, reps=5):
     ctx.synchronize()
     t_plain = time.perf_counter() - t0
     br = batched.BatchedRunner(w)
+    tuned = br.autotune()
     br.capture()
     times = []
     for _ in range(reps):
@@ -343,7 +389,8 @@ def bench_batched(rf, ctx, reps=5):
            "qps_plain": w.n_inst / t_plain, "qps_batched": w.n_inst / t_b,
            "speedup": t_plain / t_b, "rows_per_s_batched": w.N / t_b,
            "input_GBps_batched": w.bytes_per_batch() / t_b / 1e9,
-           "results_match": bool(match)}
+           "results_match": bool(match), "autotuned": tuned["choice"],
+           "autotune_timings_ms": tuned["timings_ms"], "autotune_s": tuned.get("tuning_s")}
     br.graph.close()
     return res
 
@@ -443,6 +490,9 @@ def main():
     if rf.device_count() < 1:
         raise SystemExit("bench.py: no HIP device visible")
     ctx = rf.Context(0)
+    global COPY_GBPS
+    COPY_GBPS = copy_bandwidth(rf, ctx)
+    log(f"device copy {COPY_GBPS:.0f} GB/s (read + write)")
     log("sort 2^28 u32 pairs ...")
     s = bench_sort_u32(rf, ctx, args.steps, args.warmup)
     log(f"  {s['ms_per_step']:.3f} ms/step, {s['rows_per_s'] / 1e9:.2f} G rows/s, "
@@ -488,6 +538,7 @@ def main():
                                "range, u32 value = row index) pairs",
                    "rows": 1 << 28, "key_bits": 32, "value_bytes": 4},
         "roofline": s["roofline"],
+        "device_copy_GBps": round(COPY_GBPS, 1),
         "cpu_baseline": cpu,
         "ops": ops,
     }

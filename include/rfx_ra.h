@@ -78,6 +78,13 @@ void* rfx_ctx_stream(rfx_ctx* ctx);
 /* Make sure the cached workspace holds at least `bytes`. */
 int rfx_ctx_reserve(rfx_ctx* ctx, uint64_t bytes);
 uint64_t rfx_ctx_workspace_bytes(rfx_ctx* ctx);
+/* Tile shapes of the context's operators, for an autotuner: "sort.ipt"
+ * (keys per thread of the onesweep tile: 8 or 12, for 4/8-byte keys with
+ * 0/4/8-byte values), "join.fb_tile" and "join.ex_tile" (2048 or 4096
+ * merged items per find_bounds / expansion tile).  0 restores the default.
+ * Results do not depend on the choice. */
+int rfx_ctx_tune(rfx_ctx* ctx, const char* knob, int value);
+int rfx_ctx_tuning(rfx_ctx* ctx, const char* knob, int* value);
 int rfx_ctx_synchronize(rfx_ctx* ctx);
 /* The process-wide context used by the Harmony-convention entry points. */
 rfx_ctx* rfx_default_ctx(void);
@@ -242,12 +249,13 @@ int rfx_reduce_by_key_u32(rfx_ctx* ctx, void* out_keys, uint32_t* out_vals,
 /* reduce_by_key_f64 and count_by_key over the same keys in one pass: run
  * r gets out_keys[r], out_vals[r] and its row count out_counts[r] (the
  * aggregate and COUNT(*) of one GROUP BY, as a Red Fox plan computes them
- * with set_reduce then set_count). */
+ * with set_reduce then set_count).  The outputs hold out_cap rows: *n_out
+ * counts every run, runs past out_cap are not written. */
 int rfx_reduce_count_by_key_f64(rfx_ctx* ctx, void* out_keys, double* out_vals,
-                                uint32_t* out_counts, uint64_t* n_out,
-                                const void* keys, const double* vals,
-                                const uint64_t* n, uint64_t n_cap,
-                                int key_bytes, int op);
+                                uint32_t* out_counts, uint64_t out_cap,
+                                uint64_t* n_out, const void* keys,
+                                const double* vals, const uint64_t* n,
+                                uint64_t n_cap, int key_bytes, int op);
 /* Count of each run (reduce_by_key of the generate() ones). */
 int rfx_count_by_key(rfx_ctx* ctx, void* out_keys, uint32_t* out_counts,
                      uint64_t* n_out, const void* keys, const uint64_t* n,

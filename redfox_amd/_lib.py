@@ -30,6 +30,8 @@ SIGNATURES = {
     "rfx_ctx_stream": (P, [P]),
     "rfx_ctx_reserve": (I, [P, U64]),
     "rfx_ctx_workspace_bytes": (U64, [P]),
+    "rfx_ctx_tune": (I, [P, c_char_p, I]),
+    "rfx_ctx_tuning": (I, [P, c_char_p, POINTER(c_int)]),
     "rfx_ctx_synchronize": (I, [P]),
     "rfx_default_ctx": (P, []),
     "rfx_device_count": (I, [POINTER(c_int)]),
@@ -72,7 +74,7 @@ SIGNATURES = {
     "rfx_join_expand_project": (I, [P, P, P, U64, P, U64, P, I, P, I, P, I, P, I, P, I, I, P]),
     "rfx_reduce_by_key_f64": (I, [P, P, P, P, P, P, P, U64, I, I]),
     "rfx_reduce_by_key_u32": (I, [P, P, P, P, P, P, P, U64, I]),
-    "rfx_reduce_count_by_key_f64": (I, [P, P, P, P, P, P, P, P, U64, I, I]),
+    "rfx_reduce_count_by_key_f64": (I, [P, P, P, P, U64, P, P, P, P, U64, I, I]),
     "rfx_count_by_key": (I, [P, P, P, P, P, P, U64, I]),
     "rfx_total_f64": (I, [P, P, P, U64]),
     "rfx_total_u32": (I, [P, P, P, U64]),

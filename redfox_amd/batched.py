@@ -193,7 +193,14 @@ class BatchedRunner:
                                (1, 1, 0, 64, 0)])                         # measure
         c.sort_pairs(self.key2, self.val, N, 4, self.seg_bits + GRP_BITS, 8)
         c.reduce_count_by_key_f64(self.okeys, self.osum, self.ocnt, self.nout, self.key2, self.val,
-                                  self.tot, N, 4, 0)
+                                  self.tot, N, 4, 0, out_cap=GROUPS * w.n_inst)
+
+    def autotune(self, reps: int = 2) -> dict:
+        """Choose the operators' tile shapes for this workload by timing the
+        whole plan (redfox_amd.autotune); capture() afterwards records them."""
+        from .autotune import tune
+
+        return tune(self.ctx, self._enqueue, reps=reps)
 
     def capture(self) -> None:
         """Run once eagerly (sizes the workspace), then record the graph."""

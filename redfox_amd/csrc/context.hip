@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
 
 #include "rfx_device.h"
@@ -268,6 +269,52 @@ int rfx_ctx_reserve(rfx_ctx* ctx, uint64_t bytes) {
 }
 
 uint64_t rfx_ctx_workspace_bytes(rfx_ctx* ctx) { return ctx ? ctx->ws_bytes : 0; }
+
+// Tuning knobs: name, field, allowed values (0 = the built-in default).
+static int* tune_field(rfx_ctx* ctx, const char* knob, const int** allowed, int* nallowed) {
+  static const int sort_ipt[] = {0, 8, 12};
+  static const int tiles[] = {0, 2048, 4096};
+  if (!std::strcmp(knob, "sort.ipt")) {
+    *allowed = sort_ipt;
+    *nallowed = 3;
+    return &ctx->tune_sort_ipt;
+  }
+  if (!std::strcmp(knob, "join.fb_tile")) {
+    *allowed = tiles;
+    *nallowed = 3;
+    return &ctx->tune_fb_tile;
+  }
+  if (!std::strcmp(knob, "join.ex_tile")) {
+    *allowed = tiles;
+    *nallowed = 3;
+    return &ctx->tune_ex_tile;
+  }
+  return nullptr;
+}
+
+int rfx_ctx_tune(rfx_ctx* ctx, const char* knob, int value) {
+  if (!ctx || !knob) return fail(RFX_EINVAL, "rfx_ctx_tune: null argument");
+  const int* allowed = nullptr;
+  int na = 0;
+  int* f = tune_field(ctx, knob, &allowed, &na);
+  if (!f) return fail(RFX_EINVAL, "rfx_ctx_tune: unknown knob '%s'", knob);
+  for (int i = 0; i < na; ++i)
+    if (allowed[i] == value) {
+      *f = value;
+      return RFX_OK;
+    }
+  return fail(RFX_EINVAL, "rfx_ctx_tune: %s cannot be %d", knob, value);
+}
+
+int rfx_ctx_tuning(rfx_ctx* ctx, const char* knob, int* value) {
+  if (!ctx || !knob || !value) return fail(RFX_EINVAL, "rfx_ctx_tuning: null argument");
+  const int* allowed = nullptr;
+  int na = 0;
+  int* f = tune_field(ctx, knob, &allowed, &na);
+  if (!f) return fail(RFX_EINVAL, "rfx_ctx_tuning: unknown knob '%s'", knob);
+  *value = *f;
+  return RFX_OK;
+}
 
 int rfx_ctx_synchronize(rfx_ctx* ctx) {
   if (!ctx) return fail(RFX_EINVAL, "null context");

@@ -159,7 +159,7 @@ void set_reduce(void* result_key, void* result_value, void* key, uint64_t* key_s
   const uint64_t n = read_cell("set_reduce", key_size) / kb;
   finish("set_reduce", c,
          reduce_by_key_sized(c, op, vkind, kb, result_key, result_value, key, value, key_size, kb,
-                             n, key_size, kb, value_size, vb, nullptr));
+                             n, key_size, kb, value_size, vb, nullptr, ~0ull));
 }
 
 void set_count(void* result_key, void* result_value, void* key, uint64_t* key_size, void* value,
@@ -170,7 +170,7 @@ void set_count(void* result_key, void* result_value, void* key, uint64_t* key_si
   const uint64_t n = read_cell("set_count", key_size) / kb;
   finish("set_count", c,
          reduce_by_key_sized(c, RFX_AGG_SUM, 1, kb, result_key, result_value, key, value, key_size,
-                             kb, n, key_size, kb, value_size, 4, nullptr));
+                             kb, n, key_size, kb, value_size, 4, nullptr, ~0ull));
 }
 
 void set_single_reduce(void* result, void* begin, uint64_t bytes, uint64_t kind) {

@@ -406,9 +406,9 @@ __global__ __launch_bounds__(256) void k_gather(GatherSpec gs, const int32_t* __
             compose(gs, static_cast<uint32_t>(li[g]), static_cast<uint32_t>(ri[g])));
 }
 
-template <class K>
-int find_bounds_impl(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, const void* left,
-                     uint64_t nl, const void* right, uint64_t nr) {
+template <class K, int NV>
+int find_bounds_nv(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, const void* left,
+                   uint64_t nl, const void* right, uint64_t nr) {
   if (nl == 0 || nr == 0) {
     // No matches: every count is zero and every lower bound is 0 (or nr
     // when the right side is empty... which is also 0).
@@ -419,7 +419,6 @@ int find_bounds_impl(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, co
     RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(total), 2, 0));
     return RFX_OK;
   }
-  constexpr int NV = sizeof(K) <= 8 ? kFbNV : 2048;
   const uint64_t ntiles = (nl + nr + NV - 1) / NV;
   Carve c(nullptr);
   c.take<uint64_t>(ntiles + 1);
@@ -446,31 +445,54 @@ int find_bounds_impl(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, co
   return RFX_OK;
 }
 
-int expand_impl(rfx_ctx* ctx, int mode, int32_t* li, int32_t* ri, const GatherSpec& gs,
-                const uint64_t* total, uint64_t total_cap, const int32_t* lb, const int32_t* lc,
-                uint64_t nl, const EmitSpec& es = EmitSpec{}) {
+// Tile of 2048 or 4096 merged keys (tunable for keys of up to 8 bytes;
+// 16-byte keys always take 2048).
+template <class K>
+int find_bounds_impl(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, const void* left,
+                     uint64_t nl, const void* right, uint64_t nr) {
+  if constexpr (sizeof(K) <= 8) {
+    if (ctx->tune_fb_tile == 2048 || (ctx->tune_fb_tile == 0 && kFbNV == 2048))
+      return find_bounds_nv<K, 2048>(ctx, lb, lc, total, left, nl, right, nr);
+    return find_bounds_nv<K, 4096>(ctx, lb, lc, total, left, nl, right, nr);
+  } else {
+    return find_bounds_nv<K, 2048>(ctx, lb, lc, total, left, nl, right, nr);
+  }
+}
+
+template <int NV>
+int expand_nv(rfx_ctx* ctx, int mode, int32_t* li, int32_t* ri, const GatherSpec& gs,
+              const uint64_t* total, uint64_t total_cap, const int32_t* lb, const int32_t* lc,
+              uint64_t nl, const EmitSpec& es) {
   if (nl == 0 || total_cap == 0) return RFX_OK;
-  const uint64_t ntiles = (total_cap + nl + kExNV - 1) / kExNV;
+  const uint64_t ntiles = (total_cap + nl + NV - 1) / NV;
   Carve c(nullptr);
   c.take<uint64_t>(ntiles + 1);
   RFX_TRY(ensure_workspace(ctx, c.off));
   Carve w(ctx->ws);
   uint64_t* split = w.take<uint64_t>(ntiles + 1);
   k_expand_partition<<<static_cast<uint32_t>((ntiles + 1 + 255) / 256), 256, 0, ctx->stream>>>(
-      lc, nl, total, total_cap, ntiles, kExNV, split);
+      lc, nl, total, total_cap, ntiles, NV, split);
   RFX_LAUNCHED();
   RFX_PROF(ctx, mode == 0 ? "join.expand" : mode == 1 ? "join.expand_gather" : "join.expand_project");
   if (mode == 0)
-    k_expand<0, kExNT, kExNV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
+    k_expand<0, kExNT, NV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
         lb, lc, nl, total, total_cap, split, li, ri, gs, es);
   else if (mode == 1)
-    k_expand<1, kExNT, kExNV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
+    k_expand<1, kExNT, NV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
         lb, lc, nl, total, total_cap, split, li, ri, gs, es);
   else
-    k_expand<2, kExNT, kExNV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
+    k_expand<2, kExNT, NV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
         lb, lc, nl, total, total_cap, split, li, ri, gs, es);
   RFX_LAUNCHED();
   return RFX_OK;
+}
+
+int expand_impl(rfx_ctx* ctx, int mode, int32_t* li, int32_t* ri, const GatherSpec& gs,
+                const uint64_t* total, uint64_t total_cap, const int32_t* lb, const int32_t* lc,
+                uint64_t nl, const EmitSpec& es = EmitSpec{}) {
+  const int nv = ctx->tune_ex_tile ? ctx->tune_ex_tile : kExNV;
+  if (nv == 2048) return expand_nv<2048>(ctx, mode, li, ri, gs, total, total_cap, lb, lc, nl, es);
+  return expand_nv<4096>(ctx, mode, li, ri, gs, total, total_cap, lb, lc, nl, es);
 }
 
 int check_gather(const GatherSpec& g) {

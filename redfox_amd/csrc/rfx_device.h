@@ -236,27 +236,31 @@ __device__ __forceinline__ uint64_t wave_lookback64(const uint64_t* status, uint
   uint64_t excl = 0;
   int64_t base = static_cast<int64_t>(tile);
   while (base > 0) {
-    int64_t k = base - 1 - static_cast<int64_t>(lane);
+    const int64_t k = base - 1 - static_cast<int64_t>(lane);
     uint64_t v = 0;
     bool have = k < 0;  // lanes past tile 0 count as an empty inclusive prefix
-    if (!have) {
-      v = status_load64(status + k);
-      have = (v >> 62) != 0;
-    }
-    // Wait until every lane in the window sees a published word.
-    uint64_t ready = ballot(have);
-    if (ready != ~0ull) {
+    while (true) {
+      if (!have) {
+        v = status_load64(status + k);
+        have = (v >> 62) != 0;
+      }
+      const uint64_t ready = ballot(have);
+      const bool incl = k < 0 || (v >> 62) == 2;
+      const uint64_t incl_mask = ballot(have && incl);
+      // Only the lanes up to the nearest inclusive word are needed, and
+      // they must all have published.
+      const uint32_t nr = ready == ~0ull ? 64u : static_cast<uint32_t>(__builtin_ctzll(~ready));
+      const uint32_t fi = incl_mask ? static_cast<uint32_t>(__builtin_ctzll(incl_mask)) : 64u;
+      if (fi < nr) {
+        excl += wave_sum((lane <= fi && k >= 0) ? (v & kCountMask64) : 0ull);
+        return excl;
+      }
+      if (nr == 64) {
+        excl += wave_sum(v & kCountMask64);  // 64 aggregates
+        break;
+      }
       __builtin_amdgcn_s_sleep(1);
-      continue;
     }
-    bool incl = k >= 0 && (v >> 62) == 2;
-    if (k < 0) { v = 0; incl = true; }
-    uint64_t incl_mask = ballot(incl);
-    // Lanes up to and including the nearest inclusive word contribute.
-    uint32_t stop = incl_mask ? __builtin_ctzll(incl_mask) : kWave;
-    uint64_t mine = (lane <= stop) ? (v & kCountMask64) : 0;
-    excl += wave_sum(mine);
-    if (incl_mask) break;
     base -= kWave;
   }
   return excl;

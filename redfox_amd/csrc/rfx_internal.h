@@ -26,6 +26,10 @@ struct rfx_ctx {
   bool profiling = false;
   bool capturing = false;     // inside rfx_graph_begin/end: no allocation
   bool ws_borrowed = false;   // ws belongs to the caller: never grown or freed
+  // Tile shapes chosen by rfx_ctx_tune (0: the built-in default).
+  int tune_sort_ipt = 0;      // keys per thread of the onesweep tile
+  int tune_fb_tile = 0;       // merged keys per find_bounds tile
+  int tune_ex_tile = 0;       // merged items per expansion tile
   std::vector<rfx_prof_rec> prof_pending;
   std::vector<hipEvent_t> prof_pool;
   std::map<std::string, std::pair<uint64_t, double>> prof_stats;  // launches, ms
@@ -109,11 +113,12 @@ inline int bytes_of_bits(uint64_t bits) {
 // n_cap); writes runs * out_a_mul to *out_a and runs * out_b_mul to *out_b.
 // val_kind: 0 f64 (op = RFX_AGG_*), 1 u32 sum, 2 count (u32 counts to
 // out_vals), 3 f64 with the row count of each run also to out_counts.
+// Runs past out_cap are counted but not written.
 int reduce_by_key_sized(rfx_ctx* ctx, int op, int val_kind, int kb, void* out_keys,
                         void* out_vals, const void* keys, const void* vals,
                         const uint64_t* n_ptr, uint64_t n_div, uint64_t n_cap, uint64_t* out_a,
                         uint64_t out_a_mul, uint64_t* out_b, uint64_t out_b_mul,
-                        uint32_t* out_counts);
+                        uint32_t* out_counts, uint64_t out_cap);
 // kind 0: f64 sum, 1: u32 sum, 2: max of u64 bit patterns.
 int single_reduce(rfx_ctx* ctx, int kind, void* result, const void* v, uint64_t n);
 

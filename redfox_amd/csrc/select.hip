@@ -4,9 +4,12 @@
 // select replaces the reference's three-launch form (select_field_constant
 // per-CTA compaction into a temp buffer with a CTA histogram, Select.h:30-114;
 // a 1-CTA histogram scan, Scan.h:18; getResultSize + gather,
-// Select.h:1013-1130) with one pass: each tile evaluates the predicate,
-// compacts with a block scan and takes its output offset from a decoupled
-// look-back, so rows are read once and written once, in input order.
+// Select.h:1013-1130) with one pass that streams rows through registers:
+// each wave ballots the predicate over 64 consecutive rows at a time, the
+// per-(wave, step) counts are scanned in LDS, the tile takes its output
+// offset from a decoupled look-back, and each kept row is stored straight
+// to its place (its rank inside the ballot by mbcnt).  Rows are read once
+// and written once, in input order.
 // project, split and merge are the streaming kernels of Project.h:58-130
 // and Split_Merge.h:303-531.
 #include "rfx_device.h"
@@ -16,8 +19,10 @@ namespace rfx {
 namespace {
 
 constexpr int kFNT = 256;
-constexpr int kFVT = 8;
-constexpr int kFNV = kFNT * kFVT;
+constexpr int kFNW = kFNT / kWave;
+// Rows per thread: 16, or 8 for 16-byte tuples (register budget).
+template <class K>
+constexpr int kFVT = sizeof(K) >= 16 ? 8 : 16;
 
 template <class K>
 __device__ __forceinline__ u128 widen(K v) { return to128(v); }
@@ -67,46 +72,55 @@ __global__ __launch_bounds__(kFNT) void k_select(const K* __restrict__ in, uint6
                                                  uint64_t* __restrict__ n_out,
                                                  uint64_t* __restrict__ status,
                                                  uint32_t* __restrict__ ticket) {
-  __shared__ K rows[kFNV];
-  __shared__ uint32_t scan_tmp[kFNT / kWave + 1];
+  constexpr int VT = kFVT<K>;
+  constexpr int TILE = kFNT * VT;
+  static_assert(VT * kFNW <= kWave, "one wave scans the step counts");
+  __shared__ uint32_t s_cnt[VT * kFNW + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
-  const uint64_t base = static_cast<uint64_t>(tile) * kFNV;
-  const uint32_t cnt = static_cast<uint32_t>((n - base) < kFNV ? (n - base) : kFNV);
-  for (uint32_t i = tid; i < cnt; i += kFNT) rows[i] = in[base + i];
-  __syncthreads();
-  uint32_t keepmask = 0;
-  K mine[kFVT];
+  const uint64_t base = static_cast<uint64_t>(tile) * TILE;
+  // Step j of wave w covers rows base + j*kFNT + w*64 .. +63.
+  K r[VT];
 #pragma unroll
-  for (int s = 0; s < kFVT; ++s) {
-    const uint32_t i = tid * kFVT + s;
-    if (i < cnt) {
-      mine[s] = rows[i];
-      const uint64_t f = mask128(shr128(widen(mine[s]), p.shift), p.bits).lo;
-      keepmask |= static_cast<uint32_t>(compare(f, p.bits, p.op, p.k)) << s;
-    }
+  for (int j = 0; j < VT; ++j) {
+    const uint64_t i = base + static_cast<uint64_t>(j) * kFNT + tid;
+    if (i < n) r[j] = in[i];
   }
-  uint32_t tile_kept;
-  uint32_t off = block_excl_scan<kFNT>(static_cast<uint32_t>(__popc(keepmask)), scan_tmp,
-                                       &tile_kept);
-  // rows[] is free once every thread holds its rows (the scan synchronised).
+  uint32_t keep = 0;
+  uint64_t m[VT];
 #pragma unroll
-  for (int s = 0; s < kFVT; ++s)
-    if ((keepmask >> s) & 1u) rows[off++] = mine[s];
-  if (tid < kWave) {
+  for (int j = 0; j < VT; ++j) {
+    const uint64_t i = base + static_cast<uint64_t>(j) * kFNT + tid;
+    bool k = false;
+    if (i < n) {
+      const uint64_t f = mask128(shr128(widen(r[j]), p.shift), p.bits).lo;
+      k = compare(f, p.bits, p.op, p.k);
+    }
+    keep |= static_cast<uint32_t>(k) << j;
+    m[j] = ballot(k);
+    if (lane == 0) s_cnt[j * kFNW + w] = static_cast<uint32_t>(__popcll(m[j]));
+  }
+  __syncthreads();
+  // Exclusive scan of the (step, wave) counts in row order (wave 0).
+  if (w == 0) {
+    const uint32_t c = lane < VT * kFNW ? s_cnt[lane] : 0u;
+    const uint32_t incl = wave_incl_scan(c);
+    if (lane < VT * kFNW) s_cnt[lane] = incl - c;
+    const uint32_t tile_kept = __shfl(incl, kWave - 1, kWave);
+    if (lane == 0) s_cnt[VT * kFNW] = tile_kept;
     if (tile == 0) {
-      if (tid == 0) {
+      if (lane == 0) {
         status_store64(status, kFlagIncl64 | tile_kept);
         s_prefix = 0;
       }
     } else {
-      if (tid == 0) status_store64(status + tile, kFlagAgg64 | tile_kept);
+      if (lane == 0) status_store64(status + tile, kFlagAgg64 | tile_kept);
       const uint64_t excl = wave_lookback64(status, tile);
-      if (tid == 0) {
+      if (lane == 0) {
         status_store64(status + tile, kFlagIncl64 | (excl + tile_kept));
         s_prefix = excl;
       }
@@ -114,8 +128,10 @@ __global__ __launch_bounds__(kFNT) void k_select(const K* __restrict__ in, uint6
   }
   __syncthreads();
   const uint64_t prefix = s_prefix;
-  for (uint32_t i = tid; i < tile_kept; i += kFNT) out[prefix + i] = rows[i];
-  if (tile == ntiles - 1 && tid == 0) *n_out = prefix + tile_kept;
+#pragma unroll
+  for (int j = 0; j < VT; ++j)
+    if ((keep >> j) & 1u) out[prefix + s_cnt[j * kFNW + w] + mbcnt(m[j])] = r[j];
+  if (tile == ntiles - 1 && tid == 0) *n_out = prefix + s_cnt[VT * kFNW];
 }
 
 __device__ __forceinline__ u128 load_any(const void* base, uint64_t i, int w) {
@@ -189,7 +205,8 @@ uint32_t stream_grid(rfx_ctx* ctx, uint64_t n) {
 template <class K>
 int select_impl(rfx_ctx* ctx, void* out, uint64_t* n_out, const void* in, uint64_t n, Pred p) {
   if (n == 0) return fill_u32(ctx, reinterpret_cast<uint32_t*>(n_out), 2, 0);
-  const uint64_t ntiles = (n + kFNV - 1) / kFNV;
+  constexpr uint64_t TILE = static_cast<uint64_t>(kFNT) * kFVT<K>;
+  const uint64_t ntiles = (n + TILE - 1) / TILE;
   Carve c(nullptr);
   c.take<uint64_t>(ntiles);
   c.take<uint32_t>(1);

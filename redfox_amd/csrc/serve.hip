@@ -365,8 +365,11 @@ class Plan {
       RFX_TRY(step(rfx_count_by_key(ctx_, ok2, ocnt, nout2, key2, tot, t, 4)));
       RFX_TRY(read(nout2, &g));
     } else {  // both aggregates in one pass (the same bits)
-      RFX_TRY(step(rfx_reduce_count_by_key_f64(ctx_, ok, osum, ocnt, nout, key2, val, tot, t, 4,
-                                               RFX_AGG_SUM)));
+      // The outputs hold `groups` rows: a join that broke the 1:1 rule
+      // (flagged above) leaves unwritten rows in key2 that may form more
+      // runs; those are counted but never written.
+      RFX_TRY(step(rfx_reduce_count_by_key_f64(ctx_, ok, osum, ocnt, groups, nout, key2, val, tot,
+                                               t, 4, RFX_AGG_SUM)));
     }
     RFX_TRY(finish(nout, groups));
     RFX_TRY(digest_col<4>(0, ok, groups));

@@ -8,12 +8,15 @@
 // (RelationalAlgebraKernel.cpp:919-922).
 //
 // One pass: tiles of the merge path of A and B (A wins ties) are loaded to
-// LDS; every element decides from its rank among equal values in its own
-// input and the count of equal values in the other input whether it is
-// kept (union keeps all of A and the copies of B beyond A's count;
-// difference keeps the copies of A beyond B's count; intersection the
-// copies of A within B's count); kept elements are compacted in merged
-// order, the tile offset coming from a decoupled look-back.
+// LDS.  Every element finds, by fixed-step binary searches of the tile's
+// keys run in lockstep (one element per lane), its merged position, its
+// rank among equal values in its own input and the count of equal values
+// in the other input, and from those whether it is kept (union keeps all
+// of A and the copies of B beyond A's count; difference keeps the copies
+// of A beyond B's count; intersection the copies of A within B's count).
+// Kept flags are scanned in merged order and the kept elements compacted,
+// the tile offset coming from a decoupled look-back.  Runs of equal values
+// that cross a tile edge are finished by galloping in global memory.
 #include "rfx_device.h"
 #include "rfx_internal.h"
 
@@ -40,11 +43,11 @@ __device__ __forceinline__ uint64_t mp_lower(const K* a, uint64_t na, const K* b
 template <class K>
 __global__ __launch_bounds__(256) void k_set_partition(const K* __restrict__ a, uint64_t na,
                                                        const K* __restrict__ b, uint64_t nb,
-                                                       uint64_t ntiles,
+                                                       uint64_t ntiles, uint32_t nv,
                                                        uint64_t* __restrict__ split) {
   uint64_t t = blockIdx.x * 256ull + threadIdx.x;
   if (t > ntiles) return;
-  uint64_t diag = t * kSNV;
+  uint64_t diag = t * nv;
   if (diag > na + nb) diag = na + nb;
   split[t] = mp_lower(a, na, b, nb, diag);
 }
@@ -97,74 +100,212 @@ struct Side {
   }
 };
 
-template <class K, int OP>
-__global__ __launch_bounds__(kSNT) void k_set_op(
+// Copies of x immediately before index i in the LDS array s (gallop then
+// binary search; 0 when s[i - 1] != x).
+template <class K>
+__device__ __forceinline__ uint32_t lds_run_back(const K* s, uint32_t i, K x) {
+  if (i == 0 || !eq(s[i - 1], x)) return 0;
+  uint32_t hi = i - 1, step = 1, lo;  // invariant: s[hi] == x
+  while (true) {
+    if (hi < step) {
+      lo = 0;
+      if (eq(s[0], x)) return i;
+      break;
+    }
+    lo = hi - step;
+    if (!eq(s[lo], x)) break;
+    hi = lo;
+    step <<= 1;
+  }
+  // s[lo] != x, s[hi] == x: first equal index in (lo, hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (eq(s[mid], x)) hi = mid;
+    else lo = mid;
+  }
+  return i - hi;
+}
+
+// Copies of x from index i forward in s[0, n) (0 when s[i] != x).
+template <class K>
+__device__ __forceinline__ uint32_t lds_run_fwd(const K* s, uint32_t i, uint32_t n, K x) {
+  if (i >= n || !eq(s[i], x)) return 0;
+  uint32_t lo = i, step = 1, hi;  // invariant: s[lo] == x
+  while (true) {
+    hi = lo + step;
+    if (hi >= n) {
+      if (eq(s[n - 1], x)) return n - i;
+      hi = n - 1;
+      break;
+    }
+    if (!eq(s[hi], x)) break;
+    lo = hi;
+    step <<= 1;
+  }
+  // s[lo] == x, s[hi] != x: last equal index in [lo, hi)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (eq(s[mid], x)) lo = mid;
+    else hi = mid;
+  }
+  return lo + 1 - i;
+}
+
+// Merged items per set-op tile: 4096 for keys of up to 4 bytes, fewer for
+// wider keys to bound LDS (keys + merged values + flags).
+template <class K>
+struct SetTune {
+  static constexpr int NT = 256;
+  static constexpr int NV = sizeof(K) <= 4 ? 4096 : sizeof(K) == 8 ? 2048 : 1024;
+};
+
+// Fixed-step lower (LE = false) or upper (LE = true) bounds of x[j] in the
+// sorted LDS array s[0, n), all of a thread's searches in lockstep.
+template <class K, int VT, bool LE>
+__device__ __forceinline__ void bounds(const K* s, uint32_t n, const K (&x)[VT], uint32_t jn,
+                                       uint32_t (&pos)[VT]) {
+  uint32_t top = n ? 1u << (31 - __builtin_clz(n)) : 0u;
+#pragma unroll
+  for (int j = 0; j < VT; ++j) pos[j] = 0;
+  for (uint32_t step = top; step; step >>= 1) {
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      if (static_cast<uint32_t>(j) < jn && pos[j] + step <= n) {
+        const K v = s[pos[j] + step - 1];
+        if (LE ? !lt(x[j], v) : lt(v, x[j])) pos[j] += step;
+      }
+    }
+  }
+}
+
+template <class K, int OP, int NT, int NV>
+__global__ __launch_bounds__(NT) void k_set_op(
     const K* __restrict__ a, uint64_t na, const K* __restrict__ b, uint64_t nb,
     const uint64_t* __restrict__ split, uint32_t ntiles, K* __restrict__ out,
     uint64_t* __restrict__ n_out, uint64_t n_out_mul, uint64_t* __restrict__ status,
     uint32_t* __restrict__ ticket) {
-  __shared__ K keys[kSNV];
-  __shared__ K kept[kSNV];
-  __shared__ uint32_t scan_tmp[kSNT / kWave + 1];
+  constexpr int VT = NV / NT;
+  __shared__ K keys[NV];     // the tile's A keys, then its B keys
+  __shared__ K mval[NV];     // elements by merged position, then the kept ones
+  __shared__ uint8_t mkeep[NV];
+  __shared__ uint32_t scan_tmp[NT / kWave + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
-  const uint64_t d0 = static_cast<uint64_t>(tile) * kSNV;
-  const uint64_t d1 = (d0 + kSNV < na + nb) ? d0 + kSNV : na + nb;
+  const uint64_t d0 = static_cast<uint64_t>(tile) * NV;
+  const uint64_t d1 = (d0 + NV < na + nb) ? d0 + NV : na + nb;
   const uint64_t a0 = split[tile], a1 = split[tile + 1];
   const uint64_t b0 = d0 - a0, b1 = d1 - a1;
   const uint32_t nat = static_cast<uint32_t>(a1 - a0), nbt = static_cast<uint32_t>(b1 - b0);
-  for (uint32_t i = tid; i < nat + nbt; i += kSNT)
-    keys[i] = i < nat ? a[a0 + i] : b[b0 + (i - nat)];
-  __syncthreads();
-  const Side<K> A{a, keys, na, a0, a1};
-  const Side<K> B{b, keys + nat, nb, b0, b1};
-
-  const uint32_t diag = tid * kSVT;
-  uint32_t ai = 0, bj = 0;
-  if (diag < nat + nbt) {
-    ai = static_cast<uint32_t>(mp_lower(keys, nat, keys + nat, nbt, diag));
-    bj = diag - ai;
-  }
-  K elem[kSVT];
-  uint32_t keepmask = 0;
+  const uint32_t nm = nat + nbt;
+  if constexpr (sizeof(K) <= 8) {
+    K v[VT];
 #pragma unroll
-  for (int s = 0; s < kSVT; ++s) {
-    if (diag + s >= nat + nbt) break;
-    const bool take_a = ai < nat && (bj >= nbt || !lt(keys[nat + bj], keys[ai]));
-    if (take_a) {
-      const K x = keys[ai];
-      const uint64_t gi = a0 + ai;
-      bool keep = true;
-      if (OP != RFX_SET_UNION) {
-        const uint64_t rank = A.run_bwd(gi, x);         // equal copies before it in A
-        const uint64_t cnt_b = B.run_fwd(b0 + bj, x);   // B's copies (all >= position)
-        keep = (OP == RFX_SET_DIFFERENCE) ? rank >= cnt_b : rank < cnt_b;
+    for (int j = 0; j < VT; ++j) {
+      const uint32_t i = tid + j * NT;
+      if (i < nat) v[j] = a[a0 + i];
+      else if (i < nm) v[j] = b[b0 + (i - nat)];
+    }
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      const uint32_t i = tid + j * NT;
+      if (i < nm) keys[i] = v[j];
+    }
+  } else {
+    for (uint32_t i = tid; i < nm; i += NT) keys[i] = i < nat ? a[a0 + i] : b[b0 + (i - nat)];
+  }
+  __syncthreads();
+  const K* sa = keys;
+  const K* sb = keys + nat;
+  const Side<K> GA{a, nullptr, na, 0, 0};  // global views for runs past the tile
+  const Side<K> GB{b, nullptr, nb, 0, 0};
+
+  // A elements.  Merged position = i + #{B in tile < x}.  Equal copies of
+  // x in A before a0 and in B at or after b1 are reached by galloping.
+  {
+    const uint32_t JA = (nat + NT - 1) / NT;
+    K x[VT];
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      const uint32_t i = tid + j * NT;
+      if (static_cast<uint32_t>(j) < JA) x[j] = sa[i < nat ? i : nat - 1];
+    }
+    uint32_t lbB[VT];
+    bounds<K, VT, false>(sb, nbt, x, JA, lbB);
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      const uint32_t i = tid + j * NT;
+      if (static_cast<uint32_t>(j) < JA && i < nat) {
+        bool keep = true;
+        if constexpr (OP != RFX_SET_UNION) {
+          const uint32_t before = lds_run_back(sa, i, x[j]);
+          uint64_t rank = before;
+          if (before == i && a0 > 0) rank += GA.run_bwd(a0, x[j]);
+          const uint32_t in_b = lds_run_fwd(sb, lbB[j], nbt, x[j]);
+          uint64_t cnt = in_b;
+          if (lbB[j] + in_b == nbt) cnt += GB.run_fwd(b1, x[j]);
+          keep = (OP == RFX_SET_DIFFERENCE) ? rank >= cnt : rank < cnt;
+        }
+        const uint32_t p = i + lbB[j];
+        mval[p] = x[j];
+        mkeep[p] = keep;
       }
-      elem[s] = x;
-      keepmask |= static_cast<uint32_t>(keep) << s;
-      ++ai;
-    } else {
-      const K y = keys[nat + bj];
-      const uint64_t gj = b0 + bj;
-      if (OP == RFX_SET_UNION) {
-        const uint64_t rank = B.run_bwd(gj, y);
-        const uint64_t cnt_a = A.run_bwd(a0 + ai, y);  // A's copies precede it (A wins ties)
-        elem[s] = y;
-        keepmask |= static_cast<uint32_t>(rank >= cnt_a) << s;
+    }
+  }
+  // B elements.  Merged position = j + #{A in tile <= y}.  A's copies of y
+  // before a0 and B's before b0 are reached by galloping; none of A's lie
+  // at or past a1 (they would be > y).
+  {
+    const uint32_t JB = (nbt + NT - 1) / NT;
+    K y[VT];
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      const uint32_t i = tid + j * NT;
+      if (static_cast<uint32_t>(j) < JB) y[j] = sb[i < nbt ? i : nbt - 1];
+    }
+    uint32_t ubA[VT];
+    bounds<K, VT, true>(sa, nat, y, JB, ubA);
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      const uint32_t i = tid + j * NT;
+      if (static_cast<uint32_t>(j) < JB && i < nbt) {
+        bool keep = false;
+        if constexpr (OP == RFX_SET_UNION) {
+          const uint32_t before = lds_run_back(sb, i, y[j]);
+          uint64_t rank = before;
+          if (before == i && b0 > 0) rank += GB.run_bwd(b0, y[j]);
+          const uint32_t in_a = lds_run_back(sa, ubA[j], y[j]);
+          uint64_t cnt = in_a;
+          if (in_a == ubA[j] && a0 > 0) cnt += GA.run_bwd(a0, y[j]);
+          keep = rank >= cnt;
+        }
+        const uint32_t p = i + ubA[j];
+        mval[p] = y[j];
+        mkeep[p] = keep;
       }
-      ++bj;
+    }
+  }
+  __syncthreads();
+
+  // Scan of the keep flags in merged order: thread t owns [t*VT, t*VT+VT).
+  const uint32_t c0 = tid * VT;
+  K mv[VT];
+  uint32_t kmask = 0;
+#pragma unroll
+  for (int j = 0; j < VT; ++j) {
+    if (c0 + j < nm && mkeep[c0 + j]) {
+      kmask |= 1u << j;
+      mv[j] = mval[c0 + j];
     }
   }
   uint32_t tile_kept;
-  uint32_t my_off = block_excl_scan<kSNT>(static_cast<uint32_t>(__popc(keepmask)), scan_tmp,
-                                          &tile_kept);
+  uint32_t off = block_excl_scan<NT>(static_cast<uint32_t>(__popc(kmask)), scan_tmp, &tile_kept);
 #pragma unroll
-  for (int s = 0; s < kSVT; ++s)
-    if ((keepmask >> s) & 1u) kept[my_off++] = elem[s];
+  for (int j = 0; j < VT; ++j)
+    if ((kmask >> j) & 1u) mval[off++] = mv[j];
   if (tid < kWave) {
     if (tile == 0) {
       if (tid == 0) {
@@ -182,7 +323,7 @@ __global__ __launch_bounds__(kSNT) void k_set_op(
   }
   __syncthreads();
   const uint64_t prefix = s_prefix;
-  for (uint32_t i = tid; i < tile_kept; i += kSNT) out[prefix + i] = kept[i];
+  for (uint32_t i = tid; i < tile_kept; i += NT) out[prefix + i] = mval[i];
   if (tile == ntiles - 1 && tid == 0) *n_out = (prefix + tile_kept) * n_out_mul;
 }
 
@@ -274,28 +415,29 @@ template <class K>
 int set_op_impl(rfx_ctx* ctx, int op, void* out, uint64_t* n_out, uint64_t mul, const void* a,
                 uint64_t na, const void* b, uint64_t nb) {
   if (na + nb == 0) return fill_u32(ctx, reinterpret_cast<uint32_t*>(n_out), 2, 0);
-  const uint64_t ntiles = (na + nb + kSNV - 1) / kSNV;
+  constexpr int NT = SetTune<K>::NT, NV = SetTune<K>::NV;
+  const uint64_t ntiles = (na + nb + NV - 1) / NV;
   LbWork w;
   RFX_TRY(lb_workspace(ctx, ntiles, 0, &w, nullptr));
   const K* A = static_cast<const K*>(a);
   const K* B = static_cast<const K*>(b);
   k_set_partition<K><<<static_cast<uint32_t>((ntiles + 1 + 255) / 256), 256, 0, ctx->stream>>>(
-      A, na, B, nb, ntiles, w.split);
+      A, na, B, nb, ntiles, NV, w.split);
   RFX_LAUNCHED();
   const uint32_t g = static_cast<uint32_t>(ntiles);
   K* O = static_cast<K*>(out);
   RFX_PROF(ctx, "setop");
   switch (op) {
     case RFX_SET_UNION:
-      k_set_op<K, RFX_SET_UNION><<<g, kSNT, 0, ctx->stream>>>(A, na, B, nb, w.split, g, O, n_out,
+      k_set_op<K, RFX_SET_UNION, NT, NV><<<g, NT, 0, ctx->stream>>>(A, na, B, nb, w.split, g, O, n_out,
                                                              mul, w.status, w.ticket);
       break;
     case RFX_SET_DIFFERENCE:
-      k_set_op<K, RFX_SET_DIFFERENCE><<<g, kSNT, 0, ctx->stream>>>(A, na, B, nb, w.split, g, O,
+      k_set_op<K, RFX_SET_DIFFERENCE, NT, NV><<<g, NT, 0, ctx->stream>>>(A, na, B, nb, w.split, g, O,
                                                                   n_out, mul, w.status, w.ticket);
       break;
     default:
-      k_set_op<K, RFX_SET_INTERSECTION><<<g, kSNT, 0, ctx->stream>>>(
+      k_set_op<K, RFX_SET_INTERSECTION, NT, NV><<<g, NT, 0, ctx->stream>>>(
           A, na, B, nb, w.split, g, O, n_out, mul, w.status, w.ticket);
       break;
   }

@@ -324,28 +324,42 @@ __global__ __launch_bounds__(256) void k_copyback(typename Elem<KB>::T* __restri
   }
 }
 
+// Tunable tile shapes: the autotuner may pick 8 or 12 keys per thread for
+// 4- and 8-byte keys with 0-, 4- or 8-byte values (other shapes keep the
+// default).  The smallest tile needs the most look-back status words.
 template <int KB, int VB>
-uint64_t sort_ws_bytes(uint64_t n, int key_bits) {
-  using Tn = SortTune<KB, VB>;
+constexpr bool kSortTunable = (KB == 4 || KB == 8) && (VB == 0 || VB == 4 || VB == 8);
+constexpr int kSortMinIPT = 8;
+
+uint64_t sort_ws_tile(uint64_t n, int kb, int vb, int key_bits, uint64_t tile) {
   const int passes = (key_bits + 7) / 8;
-  const uint64_t tiles = (n + Tn::TILE - 1) / Tn::TILE;
+  const uint64_t tiles = (n + tile - 1) / tile;
   Carve s(nullptr);
   SortLayout L;
-  carve_sort(s, n, KB, VB, passes, tiles, &L);
+  carve_sort(s, n, kb, vb, passes, tiles, &L);
   return s.off;
 }
 
+// Workspace for any tile shape the context may be tuned to.
 template <int KB, int VB>
+uint64_t sort_ws_bytes(uint64_t n, int key_bits) {
+  using Tn = SortTune<KB, VB>;
+  const int ipt = kSortTunable<KB, VB> && kSortMinIPT < Tn::IPT ? kSortMinIPT : Tn::IPT;
+  return sort_ws_tile(n, KB, VB, key_bits, static_cast<uint64_t>(Tn::NT) * ipt);
+}
+
+template <int KB, int VB, int IPT>
 int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
   using K = typename Elem<KB>::T;
   using V = typename Elem<VB>::T;
-  using Tn = SortTune<KB, VB>;
+  constexpr int NT = SortTune<KB, VB>::NT;
+  constexpr int TILE = NT * IPT;
   if (n < 2) return RFX_OK;
   if (n > kCountMask) return fail(RFX_ERANGE, "sort: %llu rows exceeds 2^30-1",
                                   static_cast<unsigned long long>(n));
   const int passes = (key_bits + 7) / 8;
-  const uint64_t tiles = (n + Tn::TILE - 1) / Tn::TILE;
-  RFX_TRY(ensure_workspace(ctx, sort_ws_bytes<KB, VB>(n, key_bits)));
+  const uint64_t tiles = (n + TILE - 1) / TILE;
+  RFX_TRY(ensure_workspace(ctx, sort_ws_tile(n, KB, VB, key_bits, TILE)));
   Carve c(ctx->ws);
   SortLayout L;
   carve_sort(c, n, KB, VB, passes, tiles, &L);
@@ -369,7 +383,7 @@ int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
   RFX_LAUNCHED();
   for (int p = 0; p < passes; ++p) {
     RFX_PROF(ctx, "sort.onesweep");
-    k_onesweep<KB, VB, Tn::NT, Tn::IPT><<<static_cast<uint32_t>(tiles), Tn::NT, 0, ctx->stream>>>(
+    k_onesweep<KB, VB, NT, IPT><<<static_cast<uint32_t>(tiles), NT, 0, ctx->stream>>>(
         static_cast<K*>(keys), static_cast<K*>(L.pong_keys), static_cast<V*>(vals),
         static_cast<V*>(L.pong_vals), n, p, static_cast<uint32_t>(key_bits), L.base, L.ctrs,
         L.skip, L.sel, L.status, static_cast<uint32_t>(tiles));
@@ -384,15 +398,24 @@ int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
   return RFX_OK;
 }
 
+template <int KB, int VB>
+int sort_tuned(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
+  if constexpr (kSortTunable<KB, VB>) {
+    if (ctx->tune_sort_ipt == 8) return sort_impl<KB, VB, 8>(ctx, keys, vals, n, key_bits);
+    if (ctx->tune_sort_ipt == 12) return sort_impl<KB, VB, 12>(ctx, keys, vals, n, key_bits);
+  }
+  return sort_impl<KB, VB, SortTune<KB, VB>::IPT>(ctx, keys, vals, n, key_bits);
+}
+
 template <int KB>
 int sort_dispatch_v(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits, int vb) {
   switch (vb) {
-    case 0: return sort_impl<KB, 0>(ctx, keys, nullptr, n, key_bits);
-    case 1: return sort_impl<KB, 1>(ctx, keys, vals, n, key_bits);
-    case 2: return sort_impl<KB, 2>(ctx, keys, vals, n, key_bits);
-    case 4: return sort_impl<KB, 4>(ctx, keys, vals, n, key_bits);
-    case 8: return sort_impl<KB, 8>(ctx, keys, vals, n, key_bits);
-    case 16: return sort_impl<KB, 16>(ctx, keys, vals, n, key_bits);
+    case 0: return sort_tuned<KB, 0>(ctx, keys, nullptr, n, key_bits);
+    case 1: return sort_tuned<KB, 1>(ctx, keys, vals, n, key_bits);
+    case 2: return sort_tuned<KB, 2>(ctx, keys, vals, n, key_bits);
+    case 4: return sort_tuned<KB, 4>(ctx, keys, vals, n, key_bits);
+    case 8: return sort_tuned<KB, 8>(ctx, keys, vals, n, key_bits);
+    case 16: return sort_tuned<KB, 16>(ctx, keys, vals, n, key_bits);
     default: return fail(RFX_EINVAL, "sort: value width %d", vb);
   }
 }

@@ -118,6 +118,15 @@ class Context:
     def reserve(self, nbytes: int) -> None:
         _check("rfx_ctx_reserve", lib().rfx_ctx_reserve(self.handle, nbytes))
 
+    def tune(self, knob: str, value: int) -> None:
+        """Select a tile shape (rfx_ctx_tune); results do not depend on it."""
+        _check("rfx_ctx_tune", lib().rfx_ctx_tune(self.handle, knob.encode(), int(value)))
+
+    def tuning(self, knob: str) -> int:
+        v = ctypes.c_int(0)
+        _check("rfx_ctx_tuning", lib().rfx_ctx_tuning(self.handle, knob.encode(), byref(v)))
+        return int(v.value)
+
     @property
     def workspace_bytes(self) -> int:
         return int(lib().rfx_ctx_workspace_bytes(self.handle))
@@ -251,9 +260,12 @@ class Context:
             n_cap, key_bytes))
 
     def reduce_count_by_key_f64(self, out_keys, out_vals, out_counts, n_out, keys, vals, n,
-                                n_cap, key_bytes, op=0) -> None:
+                                n_cap, key_bytes, op=0, out_cap=None) -> None:
+        """out_cap: rows the outputs hold (default n_cap); runs past it are
+        counted in n_out but not written."""
+        cap = n_cap if out_cap is None else out_cap
         _check("rfx_reduce_count_by_key_f64", lib().rfx_reduce_count_by_key_f64(
-            self.handle, out_keys.ptr, out_vals.ptr, out_counts.ptr, n_out.ptr, keys.ptr,
+            self.handle, out_keys.ptr, out_vals.ptr, out_counts.ptr, cap, n_out.ptr, keys.ptr,
             vals.ptr, n.ptr, n_cap, key_bytes, op))
 
     def total_f64(self, result, v, n) -> None:

@@ -62,3 +62,20 @@ def test_missing_library_fails_loudly(tmp_path):
 
     with pytest.raises(ImportError):
         load(str(tmp_path / "nope.so"))
+
+
+def test_context_methods_are_plain_methods():
+    """The binding's op wrappers are callables (a stray decorator would turn
+    one into a property); only workspace_bytes is a property."""
+    import inspect
+
+    from redfox_amd.rfx import Context
+
+    for name, member in inspect.getmembers(Context):
+        if name.startswith("_"):
+            continue
+        raw = inspect.getattr_static(Context, name)
+        if name == "workspace_bytes":
+            assert isinstance(raw, property)
+        else:
+            assert inspect.isfunction(raw), name

@@ -89,6 +89,29 @@ def test_sizes_and_long_runs(rf, ctx, orc, n):
     np.testing.assert_allclose(gv, ov, rtol=REL, atol=0)
 
 
+@pytest.mark.parametrize("kb", [4, 16])
+def test_run_ends_on_tile_boundaries(rf, ctx, orc, kb):
+    """Runs ending exactly at, just before and just after tile boundaries
+    (tiles hold 2048 or 4096 rows), and runs spanning many tiles."""
+    lengths = [4096, 3 * 4096, 4095, 1, 4097, 8192 - 1, 2, 70 * 4096, 4096 * 2 + 1, 5, 4096]
+    n = sum(lengths)
+    run_keys = np.arange(len(lengths), dtype=np.uint64) * np.uint64(7) + np.uint64(3)
+    v = np.repeat(run_keys, lengths)
+    if kb == 16:
+        keys = np.empty(2 * n, dtype=np.uint64)
+        keys[0::2] = v
+        keys[1::2] = v >> np.uint64(1)
+    else:
+        keys = v.astype(np.uint32)
+    vals = datagen.uniform01(77, n) * 10.0
+    gk, gv = gpu_rbk(rf, ctx, keys, vals, kb)
+    ok, ov = orc.reduce_by_key_f64(keys, vals, kb, 0)
+    assert np.array_equal(gk, ok)
+    np.testing.assert_allclose(gv, ov, rtol=REL, atol=0)
+    gk, gc = gpu_rbk(rf, ctx, keys, None, kb, kind="count")
+    assert np.array_equal(gk, ok) and gc.tolist() == lengths
+
+
 def test_runs_not_sets(rf, ctx):
     keys = np.array([1, 1, 2, 1, 1], dtype=np.uint8)
     vals = np.array([1.0, 2.0, 3.0, 4.0, 5.0])
@@ -126,7 +149,7 @@ def test_fused_sum_and_count(rf, ctx, orc, kb, groups, n):
     OK, OV, OC = rf.DeviceBuffer(keys.nbytes), rf.DeviceBuffer(8 * n), rf.DeviceBuffer(4 * n)
     nout, ncell = rf.u64_cell(5), rf.u64_cell(n)
     # Capacity above the row count: only *n rows count.
-    ctx.reduce_count_by_key_f64(OK, OV, OC, nout, K, V, ncell, n + 12345, kb, 0)
+    ctx.reduce_count_by_key_f64(OK, OV, OC, nout, K, V, ncell, n + 12345, kb, 0, out_cap=n)
     ctx.synchronize()
     g = rf.read_u64(nout)
     fk = OK.download(keys.dtype, g * kb // keys.itemsize)
@@ -140,6 +163,29 @@ def test_fused_sum_and_count(rf, ctx, orc, kb, groups, n):
     _, oc = orc.reduce_by_key_u32(keys, np.ones(n, dtype=np.uint32), kb)
     assert np.array_equal(fk, ok) and np.array_equal(fc, oc)
     np.testing.assert_allclose(fv, ov, rtol=REL, atol=0)
+
+
+def test_output_capacity_bounds_writes(rf, ctx, orc):
+    """More runs than the outputs hold: every run is counted, only the
+    first out_cap are written, nothing past the buffers changes."""
+    n, cap, guard = 50_000, 1000, 4096
+    keys = np.arange(n, dtype=np.uint32) // np.uint32(3)  # 16,667 runs
+    vals = np.ones(n)
+    K, V = rf.DeviceBuffer.from_numpy(keys), rf.DeviceBuffer.from_numpy(vals)
+    sentinel = 0xA5
+    OK = rf.DeviceBuffer.from_numpy(np.full(cap + guard, sentinel * 0x01010101, dtype=np.uint32))
+    OV = rf.DeviceBuffer.from_numpy(np.full(cap + guard, 7.0))
+    OC = rf.DeviceBuffer.from_numpy(np.full(cap + guard, sentinel, dtype=np.uint32))
+    nout, ncell = rf.u64_cell(), rf.u64_cell(n)
+    ctx.reduce_count_by_key_f64(OK, OV, OC, nout, K, V, ncell, n, 4, 0, out_cap=cap)
+    ctx.synchronize()
+    runs = (n + 2) // 3
+    assert rf.read_u64(nout) == runs
+    ok, ov, oc = OK.download(np.uint32), OV.download(np.float64), OC.download(np.uint32)
+    assert np.array_equal(ok[:cap], np.arange(cap, dtype=np.uint32))
+    assert np.all(ov[:cap] == 3.0) and np.all(oc[:cap] == 3)
+    assert np.all(ok[cap:] == sentinel * 0x01010101)
+    assert np.all(ov[cap:] == 7.0) and np.all(oc[cap:] == sentinel)
 
 
 def test_single_reductions(rf, ctx, orc):
