@@ -95,30 +95,42 @@ __global__ __launch_bounds__(256) void k_radix_hist(const typename Elem<KB>::T* 
   __syncthreads();
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
   const uint64_t gtid = blockIdx.x * 256ull + threadIdx.x;
-  // A digit shared by every active lane of the wave (sorted or
-  // low-cardinality keys) is added once by the first lane instead of 64
-  // times to the same LDS word.
   auto count = [&](K k) {
-    const uint64_t active = ballot(true);
-    for (int p = 0; p < passes; ++p) {
-      const uint32_t d = digit_of(k, 8u * p, pass_mask(p, key_bits));
-      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-      if (ballot(d != d0) == 0) {
-        if (mbcnt(active) == 0) atomicAdd(&lh[p * kBins + d], static_cast<uint32_t>(__popcll(active)));
-      } else {
-        atomicAdd(&lh[p * kBins + d], 1u);
-      }
-    }
+    for (int p = 0; p < passes; ++p)
+      atomicAdd(&lh[p * kBins + digit_of(k, 8u * p, pass_mask(p, key_bits))], 1u);
   };
   uint64_t done = 0;
   if constexpr (PER > 1) {
+    // A 16-byte vector whose keys share a digit adds once; when every lane's
+    // vector shares one digit (sorted or low-cardinality keys), the first
+    // lane adds for the whole wave instead of 64 lanes hitting one word.
     struct alignas(16) Vec { K k[PER]; };
     const Vec* kv = reinterpret_cast<const Vec*>(keys);
     const uint64_t nvec = n / PER;
     for (uint64_t v = gtid; v < nvec; v += stride) {
       const Vec x = kv[v];
+      const uint64_t active = ballot(true);
+      for (int p = 0; p < passes; ++p) {
+        const uint32_t sh = 8u * p, m = pass_mask(p, key_bits);
+        const uint32_t d = digit_of(x.k[0], sh, m);
+        bool same = true;
 #pragma unroll
-      for (int e = 0; e < PER; ++e) count(x.k[e]);
+        for (int e = 1; e < PER; ++e) same &= digit_of(x.k[e], sh, m) == d;
+        if (ballot(same) == active) {
+          const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+          if (ballot(d != d0) == 0) {
+            if (mbcnt(active) == 0)
+              atomicAdd(&lh[p * kBins + d], static_cast<uint32_t>(PER * __popcll(active)));
+            continue;
+          }
+        }
+        if (same) {
+          atomicAdd(&lh[p * kBins + d], static_cast<uint32_t>(PER));
+        } else {
+#pragma unroll
+          for (int e = 0; e < PER; ++e) atomicAdd(&lh[p * kBins + digit_of(x.k[e], sh, m)], 1u);
+        }
+      }
     }
     done = nvec * PER;
   }
