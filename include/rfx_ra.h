@@ -79,8 +79,8 @@ void* rfx_ctx_stream(rfx_ctx* ctx);
 int rfx_ctx_reserve(rfx_ctx* ctx, uint64_t bytes);
 uint64_t rfx_ctx_workspace_bytes(rfx_ctx* ctx);
 /* Tile shapes of the context's operators, for an autotuner: "sort.ipt"
- * (keys per thread of the onesweep tile: 8 or 12, for 4/8-byte keys with
- * 0/4/8-byte values), "join.fb_tile" and "join.ex_tile" (2048 or 4096
+ * (keys per thread of the onesweep tile: 8, 12 or 16, for 4/8-byte keys
+ * with 0/4/8-byte values), "join.fb_tile" and "join.ex_tile" (2048 or 4096
  * merged items per find_bounds / expansion tile).  0 restores the default.
  * Results do not depend on the choice. */
 int rfx_ctx_tune(rfx_ctx* ctx, const char* knob, int value);

@@ -22,7 +22,7 @@ from . import rfx as R
 
 # knob -> candidates, the built-in default (0) first
 KNOBS: Dict[str, Tuple[int, ...]] = {
-    "sort.ipt": (0, 8, 12),
+    "sort.ipt": (0, 8, 12, 16),
     "join.fb_tile": (0, 2048),
     "join.ex_tile": (0, 2048),
 }

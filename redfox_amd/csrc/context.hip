@@ -272,11 +272,11 @@ uint64_t rfx_ctx_workspace_bytes(rfx_ctx* ctx) { return ctx ? ctx->ws_bytes : 0;
 
 // Tuning knobs: name, field, allowed values (0 = the built-in default).
 static int* tune_field(rfx_ctx* ctx, const char* knob, const int** allowed, int* nallowed) {
-  static const int sort_ipt[] = {0, 8, 12};
+  static const int sort_ipt[] = {0, 8, 12, 16};
   static const int tiles[] = {0, 2048, 4096};
   if (!std::strcmp(knob, "sort.ipt")) {
     *allowed = sort_ipt;
-    *nallowed = 3;
+    *nallowed = 4;
     return &ctx->tune_sort_ipt;
   }
   if (!std::strcmp(knob, "join.fb_tile")) {
