@@ -473,6 +473,59 @@ def cpu_baseline_sort():
                       f"{os.cpu_count()} visible host cores (1 used)"}
 
 
+def cpu_ops_baseline():
+    """The oracle (serial C restatements of the reference algorithms, one
+    thread) on bounded samples of each op line's workload, as rows/s."""
+    import oracle
+    from redfox_amd import datagen
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    def line(rows, dt, sample):
+        return {"value": round(rows / dt, 1), "unit": "rows/s", "cores": 1, "kind": "port",
+                "sample": f"{sample}; {dt:.3f} s"}
+
+    out = {}
+    n = 1 << 22
+    k = datagen.splitmix64(SEED + 20, n)
+    v = np.arange(n, dtype=np.uint32)
+    out["sort_u64_zipf_2^27"] = line(n, timed(lambda: oracle.sort_pairs(k, v, 8, 64, 4)),
+                                     "2^22 (u64 key, u32 value) pairs, serial LSD, 8 passes")
+    for name, nr, ns, kb in (("join_65536x262144", 65536, 262144, 16),
+                             ("join_2^24x2^26", 1 << 20, 1 << 22, 20)):
+        rk, rv, sk, sv = datagen.join_tables(SEED, nr, ns, kb)
+        rk, rv = oracle.sort_pairs(rk, rv, 4, 32, 4)
+        sk, sv = oracle.sort_pairs(sk, sv, 4, 32, 4)
+
+        def core():
+            lb, lc, tot = oracle.join_find_bounds(rk, sk, 4)
+            li, ri = oracle.join_main(lb, lc, tot)
+            oracle.join_gather(16, rk, 4, rv, 4, sv, 4, li, ri, 64, 32)
+
+        sample = (f"the same {nr} x {ns} join" if nr == 65536 else
+                  f"{nr} x {ns} rows, keys in [0, 2^{kb})") + ", find_bounds + expand + gather"
+        out[name] = line(nr + ns, timed(core), sample)
+    cols = datagen.lineitem(SEED + 2, 4_000_000)
+    out["q1_60M"] = line(4_000_000, timed(lambda: oracle.q1(*cols, 2436)), "4M lineitem rows")
+    g = (datagen.splitmix64(SEED + 21, n) % np.uint64(1 << 16)).astype(np.uint32)
+    g.sort()
+    val = datagen.uniform01(SEED + 22, n) * 1e3
+    out["agg_2^28"] = line(n, timed(lambda: (oracle.reduce_by_key_f64(g, val, 4, 0),
+                                             oracle.reduce_by_key_u32(g, np.ones(n, np.uint32), 4))),
+                           "2^22 sorted rows, 2^16 groups, compensated SUM and COUNT")
+    t = datagen.splitmix64(SEED + 23, 2 * n)
+    out["select_2^27"] = line(n, timed(lambda: oracle.select(t, 16, 64, 32, 2, 1 << 31)),
+                              "2^22 16-byte tuples, half kept")
+    a = np.unique(datagen.splitmix64(SEED + 24, n))
+    b = np.unique(datagen.splitmix64(SEED + 24 + 1, n))
+    out["setop_2^27"] = line(len(a) + len(b), timed(lambda: oracle.set_op(0, a, b, 8)),
+                             "union of two 2^22 sorted u64 sets")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -521,6 +574,10 @@ def main():
     if not args.no_cpu:
         log("cpu baseline ...")
         cpu = cpu_baseline_sort()
+        if not args.no_ops:
+            for name, base in cpu_ops_baseline().items():
+                if name in ops:
+                    ops[name]["cpu_baseline"] = base
     line = {
         "metric": METRIC,
         "value": round(s["rows_per_s"], 1),

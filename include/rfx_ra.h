@@ -246,7 +246,7 @@ int rfx_reduce_by_key_u32(rfx_ctx* ctx, void* out_keys, uint32_t* out_vals,
                           uint64_t* n_out, const void* keys,
                           const uint32_t* vals, const uint64_t* n,
                           uint64_t n_cap, int key_bytes);
-/* reduce_by_key_f64 and count_by_key over the same keys in one pass: run
+/* reduce_by_key_f64 and count_by_key over the same keys at once: run
  * r gets out_keys[r], out_vals[r] and its row count out_counts[r] (the
  * aggregate and COUNT(*) of one GROUP BY, as a Red Fox plan computes them
  * with set_reduce then set_count).  The outputs hold out_cap rows: *n_out

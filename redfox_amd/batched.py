@@ -14,8 +14,8 @@ with dim_i = (dkey: a permutation of 0..d_i-1, grp: U[0, 64)) and fact_i =
   the next operator (Runtime.cpp:99-118, 1054-1314);
 * batched: all instances as one plan.  Keys are prefixed with the instance
   number (rfx_segment_keys), so one radix sort, one find_bounds, one
-  expansion, one regrouping sort and one pass of reduce-by-key (sum and
-  count together) cover every instance.  The whole sequence is captured once into a HIP graph and
+  expansion, one regrouping sort and one reduce-by-key (sum and count
+  together) cover every instance.  The whole sequence is captured once into a HIP graph and
   replayed with a single launch.  Stable sorts keep each instance's rows in
   the order the plain form sees them, so keys and counts are identical and
   FP64 sums agree to rounding.

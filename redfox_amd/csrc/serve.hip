@@ -364,7 +364,7 @@ class Plan {
       RFX_TRY(read(nout, &g));
       RFX_TRY(step(rfx_count_by_key(ctx_, ok2, ocnt, nout2, key2, tot, t, 4)));
       RFX_TRY(read(nout2, &g));
-    } else {  // both aggregates in one pass (the same bits)
+    } else {  // both aggregates from one reduce-by-key (the same bits)
       // The outputs hold `groups` rows: a join that broke the 1:1 rule
       // (flagged above) leaves unwritten rows in key2 that may form more
       // runs; those are counted but never written.
