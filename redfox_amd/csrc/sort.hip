@@ -33,7 +33,7 @@ constexpr int kBins = 256;
 #define RFX_SORT_MINW 1
 #endif
 #ifndef RFX_SORT_LBW
-#define RFX_SORT_LBW 8
+#define RFX_SORT_LBW 4
 #endif
 // Tile shape: NT threads x IPT keys per thread.  LDS per block is about
 // kMax*TILE (reorder) + TILE (digits) + 8 KB (wave counters): 16 keys per
@@ -214,13 +214,16 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
     const bool valid = idx < n;
     const uint32_t d = digit_of(keys[j], shift, dmask);
     uint64_t m = ballot(valid);
+    // Bits above a narrow last digit are zero in every lane: their ballot
+    // is empty and the update below leaves m unchanged, so all 8 bits run
+    // unconditionally.  x is all ones where the lane's bit is set, and
+    // ~(bb ^ x) selects bb there and ~bb elsewhere.
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      if ((dmask >> b) & 1u) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bb = ballot(bit);
-        m &= bit ? bb : ~bb;
-      }
+      const uint32_t x = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
+      const uint64_t bb = ballot(x != 0u);
+      const uint64_t xx = (static_cast<uint64_t>(x) << 32) | x;
+      m &= ~(bb ^ xx);
     }
     const uint32_t below = mbcnt(m);
     const uint32_t cnt = __popcll(m);
