@@ -515,6 +515,51 @@ void set_unique(void* begin, uint64_t* bytes, uint64_t type);
 int rfx_harmony_call(const char* kernel_name, const uint64_t* args,
                      uint32_t nargs);
 
+/* ---------------------------------------------------------------------- */
+/* Harmony IR programs                                                    */
+/* ---------------------------------------------------------------------- */
+/* A loaded HIR file: a u64 byte count, a KernelControlFlowGraph message and
+ * `testCount` (u64 count, Test message) records (HarmonyIR.proto;
+ * RelationalAlgebraCompiler.cpp:9502-9520 writes it, Module::read,
+ * Module.cpp:193-333, reads it).  Replaces harmony::runtime::Runtime
+ * (Runtime.cpp:65-445,1040-1314): blocks run in file order, variables are
+ * allocated before their first Out operand and freed after their last In
+ * operand, GetSize/Resize/UpdateSize act on the size words, and binary
+ * kernels run through rfx_harmony_call with operands packed as device
+ * addresses or 8-byte values per family.  Declared sizes are byte counts,
+ * as Runtime treats them.  PTX compute kernels return RFX_ENOTSUP. */
+typedef struct rfx_hir rfx_hir;
+
+typedef struct rfx_hir_summary {
+  char name[128];
+  uint32_t blocks, variables, kernels, tests;
+  uint32_t entry, exit;
+} rfx_hir_summary;
+
+/* Parse a HIR file image (no device work).  RFX_EINVAL on a malformed or
+ * inconsistent file. */
+int rfx_hir_load(rfx_hir** out, const void* bytes, uint64_t nbytes);
+void rfx_hir_destroy(rfx_hir* program);
+int rfx_hir_info(const rfx_hir* program, rfx_hir_summary* summary);
+/* The operation list of block `block` (file order) as text:
+ * "alloc <var>", "run <kernel>", "free <var>" separated by "; ". */
+int rfx_hir_schedule(const rfx_hir* program, uint32_t block, char* text, uint64_t cap);
+/* Replace an input variable's initial contents (and size) for later runs. */
+int rfx_hir_set_input(rfx_hir* program, uint32_t variable, const void* data, uint64_t bytes);
+/* Run the program on the default device; variables still allocated at the
+ * end (inputs, outputs) stay readable until the next run or destroy. */
+int rfx_hir_run(rfx_hir* program);
+int rfx_hir_variable(const rfx_hir* program, uint32_t variable, void** device_addr,
+                     uint64_t* bytes);
+int rfx_hir_read(const rfx_hir* program, uint32_t variable, void* dst, uint64_t cap,
+                 uint64_t* bytes);
+/* Run every self-test record: set its inputs, run, and compare each listed
+ * output byte for byte (size and contents).  The loader of the reference
+ * reads these records but never runs them (HirManager.cpp:76-163).
+ * `report` receives one line per failure. */
+int rfx_hir_run_tests(rfx_hir* program, uint32_t* passed, uint32_t* failed, char* report,
+                      uint64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

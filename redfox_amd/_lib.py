@@ -109,6 +109,15 @@ SIGNATURES = {
     "set_intersection": (None, [P, P, P, U64, P, U64, U64]),
     "set_unique": (None, [P, P, U64]),
     "rfx_harmony_call": (I, [c_char_p, POINTER(c_uint64), U32]),
+    "rfx_hir_load": (I, [POINTER(c_void_p), P, U64]),
+    "rfx_hir_destroy": (None, [P]),
+    "rfx_hir_info": (I, [P, P]),
+    "rfx_hir_schedule": (I, [P, U32, c_char_p, U64]),
+    "rfx_hir_set_input": (I, [P, U32, P, U64]),
+    "rfx_hir_run": (I, [P]),
+    "rfx_hir_variable": (I, [P, U32, POINTER(c_void_p), U64P]),
+    "rfx_hir_read": (I, [P, U32, P, U64, U64P]),
+    "rfx_hir_run_tests": (I, [P, POINTER(c_uint32), POINTER(c_uint32), c_char_p, U64]),
 }
 
 
