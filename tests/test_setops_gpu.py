@@ -61,11 +61,26 @@ def test_edge_sizes(rf, ctx, orc, op, na, nb):
     assert np.array_equal(gpu_set(rf, ctx, op, a, b, 4), orc.set_op(op, a, b, 4))
 
 
+def repeated(eb, values, copies):
+    v = np.repeat(np.asarray(values, dtype=np.uint64), copies)
+    if eb == 16:  # high word holds the value, low word 0
+        e = np.zeros(2 * v.size, dtype=np.uint64)
+        e[1::2] = v
+        return e
+    return v.astype(DT[eb])
+
+
+# Runs longer than a tile (3072-6144 items) on both inputs: ranks and
+# counts continue across tile edges in both directions.
+@pytest.mark.parametrize("eb", [4, 8, 16])
 @pytest.mark.parametrize("op", OPS)
-def test_heavy_duplicates_across_tiles(rf, ctx, orc, op):
-    a = np.repeat(np.arange(5, dtype=np.uint32), 7000)
-    b = np.repeat(np.arange(2, 8, dtype=np.uint32), 4000)
-    assert np.array_equal(gpu_set(rf, ctx, op, a, b, 4), orc.set_op(op, a, b, 4))
+def test_heavy_duplicates_across_tiles(rf, ctx, orc, op, eb):
+    a = repeated(eb, range(5), 7000)
+    b = repeated(eb, range(2, 8), 4000)
+    assert np.array_equal(gpu_set(rf, ctx, op, a, b, eb), orc.set_op(op, a, b, eb))
+    a = repeated(eb, [1, 2, 3], [20000, 3, 15000])
+    b = repeated(eb, [1, 3, 4], [9000, 30000, 2])
+    assert np.array_equal(gpu_set(rf, ctx, op, a, b, eb), orc.set_op(op, a, b, eb))
 
 
 @pytest.mark.parametrize("eb", [1, 4, 8, 16])
