@@ -84,6 +84,60 @@ template <class T> __device__ __forceinline__ T from128(u128 v) {
 template <> __device__ __forceinline__ u128 from128<u128>(u128 v) { return v; }
 
 // ---------------------------------------------------------------------------
+// Runs of equal keys in a sorted LDS array.
+// ---------------------------------------------------------------------------
+// Copies of x immediately before index i in the LDS array s (gallop then
+// binary search; 0 when s[i - 1] != x).
+template <class K>
+__device__ __forceinline__ uint32_t lds_run_back(const K* s, uint32_t i, K x) {
+  if (i == 0 || !eq(s[i - 1], x)) return 0;
+  uint32_t hi = i - 1, step = 1, lo;  // invariant: s[hi] == x
+  while (true) {
+    if (hi < step) {
+      lo = 0;
+      if (eq(s[0], x)) return i;
+      break;
+    }
+    lo = hi - step;
+    if (!eq(s[lo], x)) break;
+    hi = lo;
+    step <<= 1;
+  }
+  // s[lo] != x, s[hi] == x: first equal index in (lo, hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (eq(s[mid], x)) hi = mid;
+    else lo = mid;
+  }
+  return i - hi;
+}
+
+// Copies of x from index i forward in s[0, n) (0 when s[i] != x).
+template <class K>
+__device__ __forceinline__ uint32_t lds_run_fwd(const K* s, uint32_t i, uint32_t n, K x) {
+  if (i >= n || !eq(s[i], x)) return 0;
+  uint32_t lo = i, step = 1, hi;  // invariant: s[lo] == x
+  while (true) {
+    hi = lo + step;
+    if (hi >= n) {
+      if (eq(s[n - 1], x)) return n - i;
+      hi = n - 1;
+      break;
+    }
+    if (!eq(s[hi], x)) break;
+    lo = hi;
+    step <<= 1;
+  }
+  // s[lo] == x, s[hi] != x: last equal index in [lo, hi)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (eq(s[mid], x)) lo = mid;
+    else hi = mid;
+  }
+  return lo + 1 - i;
+}
+
+// ---------------------------------------------------------------------------
 // Wave-level helpers (wave64).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() {

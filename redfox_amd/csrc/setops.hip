@@ -104,57 +104,6 @@ struct Side {
   }
 };
 
-// Copies of x immediately before index i in the LDS array s (gallop then
-// binary search; 0 when s[i - 1] != x).
-template <class K>
-__device__ __forceinline__ uint32_t lds_run_back(const K* s, uint32_t i, K x) {
-  if (i == 0 || !eq(s[i - 1], x)) return 0;
-  uint32_t hi = i - 1, step = 1, lo;  // invariant: s[hi] == x
-  while (true) {
-    if (hi < step) {
-      lo = 0;
-      if (eq(s[0], x)) return i;
-      break;
-    }
-    lo = hi - step;
-    if (!eq(s[lo], x)) break;
-    hi = lo;
-    step <<= 1;
-  }
-  // s[lo] != x, s[hi] == x: first equal index in (lo, hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (eq(s[mid], x)) hi = mid;
-    else lo = mid;
-  }
-  return i - hi;
-}
-
-// Copies of x from index i forward in s[0, n) (0 when s[i] != x).
-template <class K>
-__device__ __forceinline__ uint32_t lds_run_fwd(const K* s, uint32_t i, uint32_t n, K x) {
-  if (i >= n || !eq(s[i], x)) return 0;
-  uint32_t lo = i, step = 1, hi;  // invariant: s[lo] == x
-  while (true) {
-    hi = lo + step;
-    if (hi >= n) {
-      if (eq(s[n - 1], x)) return n - i;
-      hi = n - 1;
-      break;
-    }
-    if (!eq(s[hi], x)) break;
-    lo = hi;
-    step <<= 1;
-  }
-  // s[lo] == x, s[hi] != x: last equal index in [lo, hi)
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (eq(s[mid], x)) lo = mid;
-    else hi = mid;
-  }
-  return lo + 1 - i;
-}
-
 // Set-op tile: 512 threads x 12 merged items (6 for 16-byte keys); the
 // one LDS array holds the tile's keys and then its kept items.  On 2^27 +
 // 2^27 u64 sets this measured best among 256-1024 threads x 4-16 items.
