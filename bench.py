@@ -254,6 +254,44 @@ def bench_q1(rf, ctx, iters=5):
             "kept_rows": int(res[:, 5].sum())}
 
 
+def bench_mapfilter(rf, ctx, iters=5):
+    """MAPFILTER over 2^27 lineitem rows: WHERE shipdate <= 2436, emitting
+    disc_price = price * (1 - disc) and charge = disc_price * (1 + tax)."""
+    from redfox_amd import mapfilter as M
+
+    n = 1 << 27
+    cols = [rf.DeviceBuffer(4 * n), rf.DeviceBuffer(n), rf.DeviceBuffer(n), rf.DeviceBuffer(4 * n),
+            rf.DeviceBuffer(8 * n), rf.DeviceBuffer(8 * n), rf.DeviceBuffer(8 * n)]
+    ctx.gen_lineitem(cols, n, SEED + 9)
+    ship, price, disc, tax = cols[0], cols[4], cols[5], cols[6]
+    x = M.MapFilter([(ship, "i32"), (price, "f64"), (disc, "f64"), (tax, "f64")])
+    s, p, d, t = x.cols
+    x.where(s <= 2436)
+    x.emit(p * (1 - d), "f64")
+    x.emit(p * (1 - d) * (1 + t), "f64")
+    outs = [rf.DeviceBuffer(8 * n), rf.DeviceBuffer(8 * n)]
+    kept_cell = rf.u64_cell()
+    x.run(ctx, n, outs, kept_cell)
+    ctx.synchronize()
+    ctx.profile(True)
+    times = []
+    for _ in range(iters):
+        tm = rf.Timer(ctx)
+        tm.begin()
+        x.run(ctx, n, outs, kept_cell)
+        times.append(tm.end())
+        tm.close()
+    launches, ms = ctx.profile_read("mapfilter")
+    ctx.profile(False)
+    kept = rf.read_u64(kept_cell)
+    for b in cols + outs + [kept_cell]:
+        b.close()
+    m = median(times)
+    # Every row's four columns are read once; kept rows write two doubles.
+    return {"n": n, "kept": kept, "ms": m, "rows_per_s": n / (m * 1e-3),
+            "roofline": roofline(28 * n + 16 * kept, launches, ms)}
+
+
 def bench_agg(rf, ctx, iters=5):
     """GROUP BY key: SUM(v), COUNT(*) over 2^28 rows sorted by key (u32
     keys Zipf(1.1) over 2^20 values, f64 values) in one reduce-by-key pass."""
@@ -510,6 +548,14 @@ def cpu_ops_baseline():
         out[name] = line(nr + ns, timed(core), sample)
     cols = datagen.lineitem(SEED + 2, 4_000_000)
     out["q1_60M"] = line(4_000_000, timed(lambda: oracle.q1(*cols, 2436)), "4M lineitem rows")
+    mf_cols = [cols[0], cols[4], cols[5], cols[6]]
+    mf_out = [(("mul", ("col", 1), ("sub", ("int", 1), ("col", 2))), "f64"),
+              (("mul", ("mul", ("col", 1), ("sub", ("int", 1), ("col", 2))),
+                ("add", ("int", 1), ("col", 3))), "f64")]
+    out["mapfilter_2^27"] = line(
+        4_000_000, timed(lambda: oracle.mapfilter(("le", ("col", 0), ("int", 2436)), mf_out,
+                                                  mf_cols, 4_000_000)),
+        "4M lineitem rows, the same filter and two computed columns (numpy, vectorised)")
     g = (datagen.splitmix64(SEED + 21, n) % np.uint64(1 << 16)).astype(np.uint32)
     g.sort()
     val = datagen.uniform01(SEED + 22, n) * 1e3
@@ -564,6 +610,8 @@ def main():
         ops["agg_2^28"] = bench_agg(rf, ctx)
         log("select 2^27 16-byte tuples ...")
         ops["select_2^27"] = bench_select(rf, ctx)
+        log("mapfilter 2^27 lineitem rows ...")
+        ops["mapfilter_2^27"] = bench_mapfilter(rf, ctx)
         log("set operations 2^27 + 2^27 u64 ...")
         ops["setop_2^27"] = bench_setop(rf, ctx)
         log("batched 512 join-aggregate instances ...")

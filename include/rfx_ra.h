@@ -516,6 +516,106 @@ int rfx_harmony_call(const char* kernel_name, const uint64_t* args,
                      uint32_t nargs);
 
 /* ---------------------------------------------------------------------- */
+/* MAPFILTER: fused filter / project / arithmetic over columns            */
+/* ---------------------------------------------------------------------- */
+/* Replaces the per-instance generated arithmetic and selection kernels
+ * (set_arith / set_convert, RelationalAlgebraKernel.cpp:2001-2215;
+ * set_select, :2301-2593) with one interpreted register program: ops
+ * [0, n_filter) run on every row and register keep_reg decides whether it
+ * is kept (n_filter == 0 keeps every row); ops [n_filter, n_ops) run on the
+ * kept rows, starting from cleared registers, and their OUT ops store to
+ * the output columns at the row's rank among kept rows (input order).
+ * *n_kept (device) receives the kept count.  Registers hold 64 bits: an
+ * int64 (two's complement; C division and remainder, a zero divisor
+ * gives 0) or the bits of a double (IEEE, no fused multiply-add).  The
+ * date ops follow Date.h (seconds since 1970, TPC-H years 1992-1998). */
+#define RFX_MF_MAX_COLS 16
+#define RFX_MF_MAX_OUTS 8
+#define RFX_MF_MAX_OPS 96
+#define RFX_MF_REGS 16
+
+/* Column element types; TUPLEn columns are packed tuples of n bytes whose
+ * fields LDC extracts. */
+#define RFX_MF_U8 1
+#define RFX_MF_U16 2
+#define RFX_MF_U32 3
+#define RFX_MF_U64 4
+#define RFX_MF_I8 5
+#define RFX_MF_I16 6
+#define RFX_MF_I32 7
+#define RFX_MF_I64 8
+#define RFX_MF_F32 9
+#define RFX_MF_F64 10
+#define RFX_MF_TUPLE1 11
+#define RFX_MF_TUPLE2 12
+#define RFX_MF_TUPLE4 13
+#define RFX_MF_TUPLE8 14
+#define RFX_MF_TUPLE16 15
+
+/* Opcodes: R[d] = f(R[a], R[b], imm). */
+#define RFX_MF_NOP 0
+#define RFX_MF_LDC 1       /* R[d] = column a at the row; integers widened, floats as
+                              double bits; TUPLEn: imm bits 0-7 shift, 8-15 bits,
+                              16-17 kind (0 unsigned, 1 signed, 2 double) */
+#define RFX_MF_LDI 2       /* R[d] = imm */
+#define RFX_MF_MOV 3       /* R[d] = R[a] */
+#define RFX_MF_IADD 10
+#define RFX_MF_ISUB 11
+#define RFX_MF_IMUL 12
+#define RFX_MF_IDIV 13     /* truncating; R[b] == 0 gives 0 */
+#define RFX_MF_IMOD 14     /* sign of the dividend; R[b] == 0 gives 0 */
+#define RFX_MF_FADD 20
+#define RFX_MF_FSUB 21
+#define RFX_MF_FMUL 22
+#define RFX_MF_FDIV 23
+#define RFX_MF_I2F 30      /* int64 -> double */
+#define RFX_MF_U2F 31      /* uint64 -> double */
+#define RFX_MF_F2I 32      /* double -> int64, truncating, saturating, NaN -> 0 */
+#define RFX_MF_ILT 40      /* comparisons give 1 or 0; I signed, U unsigned, F double */
+#define RFX_MF_ILE 41
+#define RFX_MF_IEQ 42
+#define RFX_MF_INE 43
+#define RFX_MF_ULT 44
+#define RFX_MF_ULE 45
+#define RFX_MF_FLT 46
+#define RFX_MF_FLE 47
+#define RFX_MF_FEQ 48
+#define RFX_MF_FNE 49
+#define RFX_MF_AND 50      /* logical */
+#define RFX_MF_OR 51
+#define RFX_MF_NOT 52      /* R[d] = R[a] == 0 */
+#define RFX_MF_ADD_MONTH 60 /* date R[a] plus imm months (AddMonth, :2053-2090) */
+#define RFX_MF_ADD_YEAR 61  /* date R[a] plus imm years (AddYear) */
+#define RFX_MF_PART_YEAR 62 /* year of date R[a] (PartYear, :2091-2123) */
+#define RFX_MF_OUT 70      /* output column b at the row's rank = R[a], converted to
+                              the column type (integers truncated, F32 rounded) */
+
+typedef struct rfx_mf_op {
+  uint8_t code, d, a, b;
+  uint32_t reserved;
+  uint64_t imm;
+} rfx_mf_op;
+
+typedef struct rfx_mf_program {
+  uint32_t n_filter, n_ops, keep_reg, reserved;
+  rfx_mf_op ops[RFX_MF_MAX_OPS];
+} rfx_mf_program;
+
+typedef struct rfx_mf_col {
+  const void* data;
+  int32_t type, reserved;
+} rfx_mf_col;
+
+typedef struct rfx_mf_out {
+  void* data;
+  int32_t type, reserved;
+} rfx_mf_out;
+
+int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, uint64_t n_rows,
+                  const rfx_mf_program* program, const rfx_mf_out* outs, int n_outs,
+                  uint64_t* n_kept);
+
+/* ---------------------------------------------------------------------- */
 /* Harmony IR programs                                                    */
 /* ---------------------------------------------------------------------- */
 /* A loaded HIR file: a u64 byte count, a KernelControlFlowGraph message and

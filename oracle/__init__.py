@@ -43,6 +43,9 @@ _SIG = {
     "rfxo_split": (None, [P, I, P, I, P, I, U64, U32]),
     "rfxo_merge": (None, [P, I, P, I, P, I, U64, U32]),
     "rfxo_q1": (None, [P, P, P, P, P, P, P, P, U64, c_int32, I]),
+    "rfxo_date2int": (U64, [I, I, I]),
+    "rfxo_int2date": (None, [U64, P, P, P]),
+    "rfxo_date_op": (None, [I, P, P, U64, ctypes.c_int64]),
 }
 
 _lib = None
@@ -213,3 +216,176 @@ def q1(shipdate, flag, status, qty, price, disc, tax, cutoff, compensated=1):
     lib().rfxo_q1(_p(out), _p(shipdate), _p(flag), _p(status), _p(qty), _p(price), _p(disc),
                   _p(tax), len(shipdate), cutoff, compensated)
     return out.reshape(6, 8)
+
+
+# ---- dates and MAPFILTER expressions -------------------------------------
+def date2int(year, month, day):
+    return int(lib().rfxo_date2int(int(year), int(month), int(day)))
+
+
+def int2date(seconds):
+    y, m, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib().rfxo_int2date(int(seconds) & 0xFFFFFFFFFFFFFFFF, ctypes.byref(y), ctypes.byref(m),
+                        ctypes.byref(d))
+    return y.value, m.value, d.value
+
+
+def date_op(op, values, k=0):
+    """op 0 AddMonth(k), 1 AddYear(k), 2 PartYear over an int64 array."""
+    v = np.ascontiguousarray(values, dtype=np.int64)
+    out = np.empty_like(v)
+    lib().rfxo_date_op(op, _p(v), _p(out), v.size, int(k))
+    return out
+
+
+_COL_KIND = {np.dtype(np.uint64): "u", np.dtype(np.float32): "f", np.dtype(np.float64): "f"}
+
+
+def _trunc_div(a, b):
+    """C division of int64 arrays: truncating; b == 0 gives 0, b == -1 negates
+    (two's complement)."""
+    a = np.asarray(a, dtype=np.int64)
+    b = np.asarray(b, dtype=np.int64)
+    with np.errstate(all="ignore"):
+        safe = np.where((b == 0) | (b == -1), 1, b)
+        # Magnitudes as uint64, so that |INT64_MIN| = 2^63 is exact.
+        ma = np.where(a < 0, (~a.view(np.uint64)) + np.uint64(1), a.view(np.uint64))
+        mb = np.where(safe < 0, (~safe.view(np.uint64)) + np.uint64(1), safe.view(np.uint64))
+        qm = ma // mb
+        q = np.where((a < 0) != (safe < 0), (~qm) + np.uint64(1), qm).view(np.int64)
+        q = np.where(b == 0, 0, q)
+        q = np.where(b == -1, (~a.view(np.uint64) + np.uint64(1)).view(np.int64), q)
+    return q.astype(np.int64)
+
+
+def _c_mod(a, b):
+    with np.errstate(all="ignore"):
+        q = _trunc_div(a, b)
+        r = a - q * np.where(b == 0, 0, b)
+        return np.where((b == 0) | (b == -1), 0, r).astype(np.int64)
+
+
+def _f2i(x):
+    x = np.asarray(x, dtype=np.float64)
+    with np.errstate(all="ignore"):
+        t = np.trunc(np.nan_to_num(x, nan=0.0))
+        out = np.where(t >= 9.223372036854775807e18, np.iinfo(np.int64).max,
+                       np.where(t <= -9.223372036854775808e18, np.iinfo(np.int64).min, 0))
+        mid = (t > -9.223372036854775808e18) & (t < 9.223372036854775807e18)
+        out = np.where(mid, np.where(mid, t, 0).astype(np.int64), out)
+    return out.astype(np.int64)
+
+
+def mapfilter_eval(tree, cols):
+    """Evaluate an expression tree over columns (numpy arrays) row by row in
+    C's semantics on 64-bit values: integers wrap, divide truncating (a zero
+    divisor gives 0), doubles round per operation; a double operand makes
+    an arithmetic or comparison a double one, an unsigned 64-bit one an
+    unsigned comparison.  Returns (values, kind) with kind "i", "u" or "f";
+    integer values as int64 bit patterns.  This restates the per-tuple C
+    expressions of set_arith and set_select (RelationalAlgebraKernel.cpp:
+    1822-1939, 2001-2215, 2301-2593) for columns."""
+    tag = tree[0]
+    with np.errstate(all="ignore"):
+        if tag == "col":
+            c = cols[tree[1]]
+            kind = _COL_KIND.get(c.dtype, "i")
+            if kind == "f":
+                return c.astype(np.float64), "f"
+            return c.astype(np.int64) if c.dtype != np.uint64 else c.view(np.int64), kind
+        if tag == "field":
+            _, idx, shift, bits, kind = tree
+            c = cols[idx]
+            if c.dtype == np.uint64 and c.ndim == 2:  # 16-byte tuples as (lo, hi) rows
+                lo, hi = c[:, 0], c[:, 1]
+                v = (lo >> np.uint64(shift)) if shift < 64 else (hi >> np.uint64(shift - 64))
+                if 0 < shift < 64:
+                    v = v | (hi << np.uint64(64 - shift))
+            else:
+                v = c.astype(np.uint64) >> np.uint64(shift)
+            if bits < 64:
+                v = v & np.uint64((1 << bits) - 1)
+            if kind == "double":
+                return v.view(np.float64), "f"
+            if kind == "signed" and bits < 64:
+                m = np.uint64(1 << (bits - 1))
+                v = (v ^ m) - m
+            return v.view(np.int64), ("u" if bits == 64 and kind == "unsigned" else "i")
+        if tag == "int":
+            return np.array(tree[1] & 0xFFFFFFFFFFFFFFFF, dtype=np.uint64).view(np.int64), "i"
+        if tag == "float":
+            return np.float64(tree[1]), "f"
+        if tag in ("add", "sub", "mul", "div", "mod"):
+            (x, kx), (y, ky) = mapfilter_eval(tree[1], cols), mapfilter_eval(tree[2], cols)
+            if "f" in (kx, ky):
+                x = _to_f(x, kx)
+                y = _to_f(y, ky)
+                return {"add": x + y, "sub": x - y, "mul": x * y, "div": x / y}[tag], "f"
+            k = "u" if "u" in (kx, ky) else "i"
+            x = np.asarray(x, dtype=np.int64)
+            y = np.asarray(y, dtype=np.int64)
+            if tag == "div":
+                return _trunc_div(x, y), k
+            if tag == "mod":
+                return _c_mod(x, y), k
+            return {"add": x + y, "sub": x - y, "mul": x * y}[tag], k
+        if tag in ("lt", "le", "gt", "ge", "eq", "ne"):
+            (x, kx), (y, ky) = mapfilter_eval(tree[1], cols), mapfilter_eval(tree[2], cols)
+            if "f" in (kx, ky):
+                x, y = _to_f(x, kx), _to_f(y, ky)
+            elif "u" in (kx, ky):
+                x = np.asarray(x, dtype=np.int64).view(np.uint64)
+                y = np.asarray(y, dtype=np.int64).view(np.uint64)
+            r = {"lt": x < y, "le": x <= y, "gt": x > y, "ge": x >= y, "eq": x == y,
+                 "ne": x != y}[tag]
+            return np.asarray(r).astype(np.int64), "i"
+        if tag in ("and", "or"):
+            (x, _), (y, _) = mapfilter_eval(tree[1], cols), mapfilter_eval(tree[2], cols)
+            r = (x != 0) & (y != 0) if tag == "and" else (x != 0) | (y != 0)
+            return np.asarray(r).astype(np.int64), "i"
+        if tag == "not":
+            x, _ = mapfilter_eval(tree[1], cols)
+            return np.asarray(x == 0).astype(np.int64), "i"
+        if tag == "i2f":
+            x, k = mapfilter_eval(tree[1], cols)
+            return _to_f(x, k), "f"
+        if tag == "f2i":
+            x, k = mapfilter_eval(tree[1], cols)
+            return (_f2i(x) if k == "f" else x), "i"
+        if tag in ("add_month", "add_year", "part_year"):
+            x, _ = mapfilter_eval(tree[1], cols)
+            n = max(len(c) for c in cols)
+            x = np.broadcast_to(np.asarray(x, dtype=np.int64), (n,))
+            op = {"add_month": 0, "add_year": 1, "part_year": 2}[tag]
+            return date_op(op, x, tree[2] if op < 2 else 0), "i"
+    raise ValueError(f"unknown node {tag}")
+
+
+def _to_f(x, k):
+    if k == "f":
+        return np.asarray(x, dtype=np.float64)
+    if k == "u":
+        return np.asarray(x, dtype=np.int64).view(np.uint64).astype(np.float64)
+    return np.asarray(x, dtype=np.int64).astype(np.float64)
+
+
+def mapfilter(filter_tree, outputs, cols, n):
+    """Kept-row mask and the output columns (numpy, by output type) of a
+    MAPFILTER: the rows whose filter is nonzero, in input order."""
+    keep = np.ones(n, dtype=bool)
+    if filter_tree is not None:
+        f, _ = mapfilter_eval(filter_tree, cols)
+        keep = np.broadcast_to(np.asarray(f) != 0, (n,)).copy()
+    outs = []
+    np_t = {"u8": np.uint8, "u16": np.uint16, "u32": np.uint32, "u64": np.uint64, "i8": np.int8,
+            "i16": np.int16, "i32": np.int32, "i64": np.int64, "f32": np.float32, "f64": np.float64}
+    for tree, t in outputs:
+        v, k = mapfilter_eval(tree, cols)
+        v = np.broadcast_to(v, (n,))[keep]
+        if t in ("f32", "f64"):
+            v = _to_f(v, k).astype(np_t[t])
+        else:  # integers keep their low bits
+            bits = (_f2i(v) if k == "f" else np.asarray(v, dtype=np.int64)).view(np.uint64)
+            v = bits.astype(np.dtype(f"u{np.dtype(np_t[t]).itemsize}")).view(np_t[t])
+        outs.append(np.ascontiguousarray(v))
+    return keep, outs

@@ -528,3 +528,82 @@ void rfxo_q1(double* out, const int32_t* shipdate, const uint8_t* flag,
     out[g * 8 + 7] = 0.0;
   }
 }
+
+/* ---- dates ------------------------------------------------------------ */
+/* wrapdate (Date.h:17-25): months past 12 carry into the year, and the
+ * month becomes month % 12 (so a multiple of 12 gives month 0). */
+static void wrapdate(int* year, int* month) {
+  if (*month > 12) {
+    int inc_year = (*month - 1) / 12;
+    *year += inc_year;
+    *month = *month % 12;
+  }
+}
+
+/* int2date (Date.h:27-90): days since 1992-01-01, then at most six year
+ * steps (366, 365, 365, 365, 366, 365), then the month from cumulative
+ * day counts (one more from March on in 1992 and 1996). */
+void rfxo_int2date(uint64_t seconds, int* year, int* month, int* day) {
+  uint64_t days = seconds / 86400 - 8035;
+  unsigned year_int = 1992, month_int = 0;
+  static const unsigned steps[6] = {366, 365, 365, 365, 366, 365};
+  for (int s = 0; s < 6; ++s) {
+    if (days >= steps[s]) {
+      days -= steps[s];
+      year_int++;
+    } else {
+      break;
+    }
+  }
+  int month_days[13] = {0, 31, 59, 90, 120, 151, 181, 212, 243, 273, 304, 334, 365};
+  if (year_int == 1992 || year_int == 1996)
+    for (int i = 2; i < 13; ++i) month_days[i]++;
+  for (int i = 1; i < 13; ++i) {
+    if (days < (uint64_t)month_days[i] && days >= (uint64_t)month_days[i - 1]) {
+      month_int = (unsigned)i;
+      days -= (uint64_t)month_days[i - 1];
+      break;
+    }
+  }
+  *year = (int)year_int;
+  *month = (int)month_int;
+  *day = (int)(days + 1);
+}
+
+/* date2int (Date.h:92-118): 365-day years from 1970, months by fixed
+ * lengths (February 28), plus the leap days of 1972-1988 and 1992/1996
+ * by the cases Date.h lists. */
+uint64_t rfxo_date2int(int year_int, int month_int, int date_int) {
+  uint64_t days = (uint64_t)(int64_t)((year_int - 1970) * 365);
+  static const int len[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+  for (int m = 1; m <= 11; ++m)
+    if (month_int > m) days += (uint64_t)len[m - 1];
+  days += (uint64_t)(int64_t)(date_int - 1);
+  if (year_int == 1992 && month_int <= 2) days += 5;
+  if (year_int == 1992 && month_int > 2) days += 6;
+  if (year_int > 1992 && year_int < 1996) days += 6;
+  if (year_int == 1996 && month_int <= 2) days += 6;
+  if (year_int == 1996 && month_int > 2) days += 7;
+  if (year_int > 1996) days += 7;
+  return days * 86400;
+}
+
+/* The date expressions of _arithSource (RelationalAlgebraKernel.cpp:
+ * 2053-2123): AddMonth converts, adds to the month, wraps and converts
+ * back; AddYear adds to the year; PartYear returns the year. */
+void rfxo_date_op(int op, const uint64_t* in, uint64_t* out, uint64_t n, int64_t k) {
+  for (uint64_t i = 0; i < n; ++i) {
+    int year, month, day;
+    rfxo_int2date(in[i], &year, &month, &day);
+    if (op == 0) {
+      month += (int)k;
+      wrapdate(&year, &month);
+      out[i] = rfxo_date2int(year, month, day);
+    } else if (op == 1) {
+      year += (int)k;
+      out[i] = rfxo_date2int(year, month, day);
+    } else {
+      out[i] = (uint64_t)(int64_t)year;
+    }
+  }
+}

@@ -125,6 +125,16 @@ void rfxo_q1(double* out, const int32_t* shipdate, const uint8_t* flag,
              const double* disc, const double* tax, uint64_t n,
              int32_t cutoff, int compensated);
 
+/* ---- dates (redfox/redfox/ra/interface/Date.h:17-118) --------------- */
+/* Seconds since 1970 <-> (year, month, day) as Date.h computes them for the
+ * TPC-H years; the three date operations of set_arith
+ * (RelationalAlgebraKernel.cpp:2053-2123) over n values:
+ * op 0 AddMonth by k, 1 AddYear by k, 2 PartYear. */
+uint64_t rfxo_date2int(int year, int month, int day);
+void rfxo_int2date(uint64_t seconds, int* year, int* month, int* day);
+void rfxo_date_op(int op, const uint64_t* in, uint64_t* out, uint64_t n, int64_t k);
+
+
 #ifdef __cplusplus
 }
 #endif

@@ -202,7 +202,6 @@ __global__ __launch_bounds__(NT) void k_set_op(
   }
   K mv[VT];
   uint32_t kmask = 0;
-#pragma unroll
   for (int k = 0; k < VT; ++k) {
     if (diag + k >= nm) break;
     const bool take_a = bi >= nbt || (ai < nat && !lt(sb[bi], sa[ai]));
