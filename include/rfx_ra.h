@@ -81,8 +81,10 @@ uint64_t rfx_ctx_workspace_bytes(rfx_ctx* ctx);
 /* Tile shapes of the context's operators, for an autotuner: "sort.ipt"
  * (keys per thread of the onesweep tile: 8, 12 or 16, for 4/8-byte keys
  * with 0/4/8-byte values), "join.fb_tile" and "join.ex_tile" (2048 or 4096
- * merged items per find_bounds / expansion tile).  0 restores the default.
- * Results do not depend on the choice. */
+ * merged items per find_bounds / expansion tile), "mapfilter.mode" (1: run
+ * programs compiled for their types only, 2: interpret them only; the
+ * default 0 compiles and interprets where compiling is not possible).
+ * 0 restores the default.  Results do not depend on the choice. */
 int rfx_ctx_tune(rfx_ctx* ctx, const char* knob, int value);
 int rfx_ctx_tuning(rfx_ctx* ctx, const char* knob, int* value);
 int rfx_ctx_synchronize(rfx_ctx* ctx);

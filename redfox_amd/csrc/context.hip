@@ -274,6 +274,7 @@ uint64_t rfx_ctx_workspace_bytes(rfx_ctx* ctx) { return ctx ? ctx->ws_bytes : 0;
 static int* tune_field(rfx_ctx* ctx, const char* knob, const int** allowed, int* nallowed) {
   static const int sort_ipt[] = {0, 8, 12, 16};
   static const int tiles[] = {0, 2048, 4096};
+  static const int mf_modes[] = {0, 1, 2};
   if (!std::strcmp(knob, "sort.ipt")) {
     *allowed = sort_ipt;
     *nallowed = 4;
@@ -288,6 +289,11 @@ static int* tune_field(rfx_ctx* ctx, const char* knob, const int** allowed, int*
     *allowed = tiles;
     *nallowed = 3;
     return &ctx->tune_ex_tile;
+  }
+  if (!std::strcmp(knob, "mapfilter.mode")) {
+    *allowed = mf_modes;
+    *nallowed = 3;
+    return &ctx->tune_mf_mode;
   }
   return nullptr;
 }

@@ -5,148 +5,32 @@
 // and selection (set_arith / set_convert, RelationalAlgebraKernel.cpp:
 // 2001-2215: one new field computed from a C expression over the source
 // tuple's fields; set_select, :2301-2593: a comparison filter with
-// per-CTA compaction) with one kernel that interprets a small register
-// program.  Every lane runs the same program, so its opcode branches are
-// uniform and cost no divergence; the register file is indexed with
-// uniform indices (VGPR indexing mode, no scratch).
+// per-CTA compaction).  A program is a short list of register ops
+// (include/rfx_ra.h); it runs in one of two ways, with the same tile
+// (mapfilter_dev.h: filter, ballot compaction with look-back, outputs for
+// the kept rows in input order) and the same per-op semantics (mf_apply):
 //
-// One pass per tile of 256 x 8 rows, in two phases.  The filter program
-// runs on every row; each wave ballots the keep flags of 64 consecutive
-// rows per step, the (step, wave) counts are scanned in LDS, and the tile
-// takes its output offset from a decoupled look-back.  The output program
-// then runs on the kept rows only and stores each result straight to its
-// place, so the output keeps input order.
-//
-// Arithmetic: integers in two's-complement 64 bits, C division and
-// remainder (truncating; a zero divisor gives 0); doubles in IEEE binary64
-// with no contraction into fused multiply-adds, so results equal a serial
-// evaluation of the same expression.  The date operations restate
-// Date.h:17-118 (seconds since 1970 for the TPC-H years, wrapdate's month
-// arithmetic included).
-#include "rfx_device.h"
-#include "rfx_internal.h"
+//  * compiled: the program, with its column and output types, is turned
+//    into the source of one kernel, compiled with hiprtc for gfx950 on its
+//    first use and cached for the process.  Every op becomes one statement
+//    on named registers, so a program costs what the same expression
+//    written by hand costs;
+//  * interpreted: one kernel runs any program, every lane stepping through
+//    the same ops (uniform branches, registers indexed with uniform
+//    indices).  It serves while a graph is being captured, when the
+//    compiler is unavailable, or when the "mapfilter.mode" knob asks for it.
+#include <hip/hiprtc.h>
 
-#pragma clang fp contract(off)
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "mapfilter_dev.h"
+#include "rfx_internal.h"
 
 namespace rfx {
 namespace {
-
-constexpr int kMNT = 256;
-constexpr int kMNW = kMNT / kWave;
-constexpr int kMVT = 8;
-constexpr int kMTile = kMNT * kMVT;
-
-struct MfIn {
-  const void* p[RFX_MF_MAX_COLS];
-  int32_t t[RFX_MF_MAX_COLS];
-};
-struct MfOut {
-  void* p[RFX_MF_MAX_OUTS];
-  int32_t t[RFX_MF_MAX_OUTS];
-};
-
-__device__ __forceinline__ double as_f(uint64_t v) {
-  return __longlong_as_double(static_cast<long long>(v));
-}
-__device__ __forceinline__ uint64_t as_u(double d) {
-  return static_cast<uint64_t>(__double_as_longlong(d));
-}
-
-// Element i of a column of type t at p: integers widened (zero- or sign-), floats to double bits.  Packed
-// tuple columns yield the field [shift, shift + bits) named by the op's
-// immediate (bits 0-7 shift, 8-15 bits, 16-17: 0 unsigned, 1 signed,
-// 2 double).
-__device__ __forceinline__ uint64_t load_elem(const void* p, int t, uint64_t i, uint64_t imm) {
-  switch (t) {
-    case RFX_MF_U8: return static_cast<const uint8_t*>(p)[i];
-    case RFX_MF_U16: return static_cast<const uint16_t*>(p)[i];
-    case RFX_MF_U32: return static_cast<const uint32_t*>(p)[i];
-    case RFX_MF_U64: return static_cast<const uint64_t*>(p)[i];
-    case RFX_MF_I8: return static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int8_t*>(p)[i]));
-    case RFX_MF_I16: return static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int16_t*>(p)[i]));
-    case RFX_MF_I32: return static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(p)[i]));
-    case RFX_MF_I64: return static_cast<const uint64_t*>(p)[i];
-    case RFX_MF_F32: return as_u(static_cast<double>(static_cast<const float*>(p)[i]));
-    case RFX_MF_F64: return static_cast<const uint64_t*>(p)[i];
-    default: break;
-  }
-  u128 x;
-  switch (t) {
-    case RFX_MF_TUPLE1: x = {static_cast<const uint8_t*>(p)[i], 0}; break;
-    case RFX_MF_TUPLE2: x = {static_cast<const uint16_t*>(p)[i], 0}; break;
-    case RFX_MF_TUPLE4: x = {static_cast<const uint32_t*>(p)[i], 0}; break;
-    case RFX_MF_TUPLE8: x = {static_cast<const uint64_t*>(p)[i], 0}; break;
-    default: x = static_cast<const u128*>(p)[i]; break;
-  }
-  const uint32_t shift = imm & 0xFF, bits = (imm >> 8) & 0xFF, kind = (imm >> 16) & 3;
-  const uint64_t f = mask128(shr128(x, shift), bits).lo;
-  if (kind == 1 && bits < 64) {
-    const uint64_t m = 1ull << (bits - 1);
-    return (f ^ m) - m;
-  }
-  return f;  // unsigned, or the bits of a double when kind == 2
-}
-
-__device__ __forceinline__ void store_out(const MfOut& out, uint32_t k, uint64_t pos, uint64_t v) {
-  void* p = out.p[k];
-  switch (out.t[k]) {
-    case RFX_MF_U8: case RFX_MF_I8: static_cast<uint8_t*>(p)[pos] = static_cast<uint8_t>(v); break;
-    case RFX_MF_U16: case RFX_MF_I16: static_cast<uint16_t*>(p)[pos] = static_cast<uint16_t>(v); break;
-    case RFX_MF_U32: case RFX_MF_I32: static_cast<uint32_t*>(p)[pos] = static_cast<uint32_t>(v); break;
-    case RFX_MF_F32: static_cast<float*>(p)[pos] = static_cast<float>(as_f(v)); break;
-    default: static_cast<uint64_t*>(p)[pos] = v; break;  // U64, I64, F64
-  }
-}
-
-// Date.h: seconds since 1970-01-01 <-> (year, month, day) for 1992-1998.
-__device__ void int2date(uint64_t seconds, int& year, int& month, int& day) {
-  uint64_t days = seconds / 86400 - 8035;  // days since 1992-01-01
-  int y = 1992;
-  // Year lengths from 1992 on; the walk stops at the first that does not fit
-  // (six steps at most, as Date.h:37-88 does).
-  const uint32_t len[6] = {366, 365, 365, 365, 366, 365};
-  for (int k = 0; k < 6 && days >= len[k]; ++k) {
-    days -= len[k];
-    ++y;
-  }
-  uint32_t start[13] = {0, 31, 59, 90, 120, 151, 181, 212, 243, 273, 304, 334, 365};
-  if (y == 1992 || y == 1996)
-    for (int k = 2; k < 13; ++k) ++start[k];
-  int m = 0;
-  for (int k = 1; k < 13; ++k) {
-    if (days < start[k] && days >= start[k - 1]) {
-      m = k;
-      days -= start[k - 1];
-      break;
-    }
-  }
-  year = y;
-  month = m;
-  day = static_cast<int>(days) + 1;
-}
-
-__device__ uint64_t date2int(int year, int month, int day) {
-  // Cumulative days before each month (non-leap); leap days only through
-  // the 1992 and 1996 adjustments below, as Date.h:92-118 counts them.
-  const uint32_t before[13] = {0, 0, 31, 59, 90, 120, 151, 181, 212, 243, 273, 304, 334};
-  uint64_t days = static_cast<uint64_t>(static_cast<int64_t>(year - 1970) * 365);
-  days += before[month < 1 ? 0 : month > 12 ? 12 : month];
-  days += static_cast<uint64_t>(static_cast<int64_t>(day - 1));
-  if (year == 1992 && month <= 2) days += 5;
-  if (year == 1992 && month > 2) days += 6;
-  if (year > 1992 && year < 1996) days += 6;
-  if (year == 1996 && month <= 2) days += 6;
-  if (year == 1996 && month > 2) days += 7;
-  if (year > 1996) days += 7;
-  return days * 86400;
-}
-
-__device__ __forceinline__ void wrapdate(int& year, int& month) {
-  if (month > 12) {
-    year += (month - 1) / 12;
-    month = month % 12;
-  }
-}
 
 // Run ops [begin, end) for row r.  OUT ops store to position `pos`.
 // Returns the keep register.
@@ -158,138 +42,29 @@ __device__ __forceinline__ uint64_t run_ops(const rfx_mf_program& prog, uint32_t
   for (int k = 0; k < RFX_MF_REGS; ++k) R[k] = 0;
   for (uint32_t pc = begin; pc < end; ++pc) {
     const rfx_mf_op op = prog.ops[pc];
-    const uint64_t a = R[op.a], b = R[op.b];
-    const int64_t sa = static_cast<int64_t>(a), sb = static_cast<int64_t>(b);
-    uint64_t v = 0;
-    switch (op.code) {
-      case RFX_MF_LDC: v = load_elem(in.p[op.a], in.t[op.a], r, op.imm); break;
-      case RFX_MF_LDI: v = op.imm; break;
-      case RFX_MF_MOV: v = a; break;
-      case RFX_MF_IADD: v = a + b; break;
-      case RFX_MF_ISUB: v = a - b; break;
-      case RFX_MF_IMUL: v = a * b; break;
-      case RFX_MF_IDIV:
-        v = sb == 0 ? 0 : (sb == -1 ? 0 - a : static_cast<uint64_t>(sa / sb));
-        break;
-      case RFX_MF_IMOD: v = (sb == 0 || sb == -1) ? 0 : static_cast<uint64_t>(sa % sb); break;
-      case RFX_MF_FADD: v = as_u(as_f(a) + as_f(b)); break;
-      case RFX_MF_FSUB: v = as_u(as_f(a) - as_f(b)); break;
-      case RFX_MF_FMUL: v = as_u(as_f(a) * as_f(b)); break;
-      case RFX_MF_FDIV: v = as_u(as_f(a) / as_f(b)); break;
-      case RFX_MF_I2F: v = as_u(static_cast<double>(sa)); break;
-      case RFX_MF_U2F: v = as_u(static_cast<double>(a)); break;
-      case RFX_MF_F2I: {
-        const double d = as_f(a);
-        v = !(d == d) ? 0
-            : d >= 9223372036854775807.0 ? static_cast<uint64_t>(INT64_MAX)
-            : d <= -9223372036854775808.0 ? static_cast<uint64_t>(INT64_MIN)
-                                          : static_cast<uint64_t>(static_cast<int64_t>(d));
-        break;
-      }
-      case RFX_MF_ILT: v = sa < sb; break;
-      case RFX_MF_ILE: v = sa <= sb; break;
-      case RFX_MF_IEQ: v = a == b; break;
-      case RFX_MF_INE: v = a != b; break;
-      case RFX_MF_ULT: v = a < b; break;
-      case RFX_MF_ULE: v = a <= b; break;
-      case RFX_MF_FLT: v = as_f(a) < as_f(b); break;
-      case RFX_MF_FLE: v = as_f(a) <= as_f(b); break;
-      case RFX_MF_FEQ: v = as_f(a) == as_f(b); break;
-      case RFX_MF_FNE: v = as_f(a) != as_f(b); break;
-      case RFX_MF_AND: v = (a != 0) && (b != 0); break;
-      case RFX_MF_OR: v = (a != 0) || (b != 0); break;
-      case RFX_MF_NOT: v = a == 0; break;
-      case RFX_MF_ADD_MONTH:
-      case RFX_MF_ADD_YEAR: {
-        int y, m, d;
-        int2date(a, y, m, d);
-        if (op.code == RFX_MF_ADD_MONTH) {
-          m += static_cast<int>(static_cast<int64_t>(op.imm));
-          wrapdate(y, m);
-        } else {
-          y += static_cast<int>(static_cast<int64_t>(op.imm));
-        }
-        v = date2int(y, m, d);
-        break;
-      }
-      case RFX_MF_PART_YEAR: {
-        int y, m, d;
-        int2date(a, y, m, d);
-        v = static_cast<uint64_t>(y);
-        break;
-      }
-      case RFX_MF_OUT:
-        store_out(out, op.b, pos, a);
-        continue;
-      default:
-        continue;
+    if (op.code == RFX_MF_LDC) {
+      R[op.d] = load_elem(in.p[op.a], in.t[op.a], r, op.imm);
+    } else if (op.code == RFX_MF_OUT) {
+      store_out(out.p[op.b], out.t[op.b], pos, R[op.a]);
+    } else if (op.code != RFX_MF_NOP) {
+      R[op.d] = mf_apply(op.code, R[op.a], R[op.b], op.imm);
     }
-    R[op.d] = v;
   }
   return R[prog.keep_reg];
 }
 
+// The row loops stay rolled: each holds one copy of the interpreter.
 __global__ __launch_bounds__(kMNT) void k_mapfilter(MfIn in, uint64_t n, rfx_mf_program prog,
                                                     MfOut out, uint32_t ntiles,
                                                     uint64_t* __restrict__ n_kept,
                                                     uint64_t* __restrict__ status,
                                                     uint32_t* __restrict__ ticket) {
-  __shared__ uint32_t s_cnt[kMVT * kMNW + 1];
-  __shared__ uint64_t s_ball[kMVT * kMNW];
-  __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_prefix;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
-  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint64_t base = static_cast<uint64_t>(tile) * kMTile;
-  // Step j of wave w covers rows base + j*256 + w*64 .. +63.  The row
-  // loops stay rolled: each holds one copy of the interpreter.
-  uint32_t keep = 0;
-#pragma unroll 1
-  for (int j = 0; j < kMVT; ++j) {
-    const uint64_t r = base + static_cast<uint64_t>(j) * kMNT + tid;
-    bool k = false;
-    if (r < n) k = prog.n_filter == 0 || run_ops(prog, 0, prog.n_filter, in, out, r, 0) != 0;
-    keep |= static_cast<uint32_t>(k) << j;
-    const uint64_t m = ballot(k);
-    if (lane == 0) {
-      s_cnt[j * kMNW + w] = static_cast<uint32_t>(__popcll(m));
-      s_ball[j * kMNW + w] = m;
-    }
-  }
-  __syncthreads();
-  if (w == 0) {
-    const uint32_t c = lane < kMVT * kMNW ? s_cnt[lane] : 0u;
-    const uint32_t incl = wave_incl_scan(c);
-    if (lane < kMVT * kMNW) s_cnt[lane] = incl - c;
-    const uint32_t tile_kept = __shfl(incl, kWave - 1, kWave);
-    if (lane == 0) s_cnt[kMVT * kMNW] = tile_kept;
-    if (tile == 0) {
-      if (lane == 0) {
-        status_store64(status, kFlagIncl64 | tile_kept);
-        s_prefix = 0;
-      }
-    } else {
-      if (lane == 0) status_store64(status + tile, kFlagAgg64 | tile_kept);
-      const uint64_t excl = wave_lookback64(status, tile);
-      if (lane == 0) {
-        status_store64(status + tile, kFlagIncl64 | (excl + tile_kept));
-        s_prefix = excl;
-      }
-    }
-  }
-  __syncthreads();
-  const uint64_t prefix = s_prefix;
-#pragma unroll 1
-  for (int j = 0; j < kMVT; ++j) {
-    if ((keep >> j) & 1u) {
-      const uint64_t r = base + static_cast<uint64_t>(j) * kMNT + tid;
-      run_ops(prog, prog.n_filter, prog.n_ops, in, out, r,
-              prefix + s_cnt[j * kMNW + w] + mbcnt(s_ball[j * kMNW + w]));
-    }
-  }
-  if (tile == ntiles - 1 && tid == 0) *n_kept = prefix + s_cnt[kMVT * kMNW];
+  mapfilter_tile<1>(
+      [&](uint64_t r) {
+        return prog.n_filter == 0 || run_ops(prog, 0, prog.n_filter, in, out, r, 0) != 0;
+      },
+      [&](uint64_t r, uint64_t pos) { run_ops(prog, prog.n_filter, prog.n_ops, in, out, r, pos); },
+      n, ntiles, n_kept, status, ticket);
 }
 
 bool valid_col_type(int t) { return t >= RFX_MF_U8 && t <= RFX_MF_TUPLE16; }
@@ -299,6 +74,160 @@ bool valid_code(uint32_t c) {
          (c >= RFX_MF_FADD && c <= RFX_MF_FDIV) || (c >= RFX_MF_I2F && c <= RFX_MF_F2I) ||
          (c >= RFX_MF_ILT && c <= RFX_MF_FNE) || (c >= RFX_MF_AND && c <= RFX_MF_NOT) ||
          (c >= RFX_MF_ADD_MONTH && c <= RFX_MF_PART_YEAR) || c == RFX_MF_OUT;
+}
+
+// ---------------------------------------------------------------------------
+// Compiled programs.
+// ---------------------------------------------------------------------------
+
+// What the run-time compiler sees: the device headers this library is
+// built from (embedded at build time) and stand-ins for the system headers
+// they name; hiprtc supplies the HIP device API and the fixed-width
+// integer types (in __hip_internal) itself.
+constexpr const char kStdintStub[] =
+    "#pragma once\n"
+    "using __hip_internal::int8_t; using __hip_internal::int16_t;\n"
+    "using __hip_internal::int32_t; using __hip_internal::int64_t;\n"
+    "using __hip_internal::uint8_t; using __hip_internal::uint16_t;\n"
+    "using __hip_internal::uint32_t; using __hip_internal::uint64_t;\n"
+    "#ifndef INT64_MAX\n#define INT64_MAX 9223372036854775807LL\n#endif\n"
+    "#ifndef INT64_MIN\n#define INT64_MIN (-INT64_MAX - 1LL)\n#endif\n";
+
+struct JitHeader {
+  const char* name;
+  const char* text;
+};
+const JitHeader kJitHeaders[] = {
+#include "mf_jit_headers.inc"
+    {"hip/hip_runtime.h", "#pragma once\n"},
+    {"cstdint", kStdintStub},
+    {"stdint.h", kStdintStub},
+};
+
+// Kernels compiled so far, by device and source.  Modules stay loaded for
+// the life of the process.
+std::mutex g_jit_mu;
+std::unordered_map<std::string, hipFunction_t> g_jit_fns;
+
+constexpr const char kJitName[] = "rfx_mf_jit";
+
+void jit_ops(std::string& s, const rfx_mf_program& p, uint32_t begin, uint32_t end,
+             const rfx_mf_col* cols, const rfx_mf_out* outs) {
+  char line[160];
+  for (uint32_t pc = begin; pc < end; ++pc) {
+    const rfx_mf_op& op = p.ops[pc];
+    if (op.code == RFX_MF_NOP) continue;
+    if (op.code == RFX_MF_LDC)
+      std::snprintf(line, sizeof line, "    R[%u] = load_elem(in.p[%u], %d, row, 0x%llxull);\n",
+                    op.d, op.a, cols[op.a].type, static_cast<unsigned long long>(op.imm));
+    else if (op.code == RFX_MF_OUT)
+      std::snprintf(line, sizeof line, "    store_out(out.p[%u], %d, pos, R[%u]);\n", op.b,
+                    outs[op.b].type, op.a);
+    else
+      std::snprintf(line, sizeof line, "    R[%u] = mf_apply(%u, R[%u], R[%u], 0x%llxull);\n",
+                    op.d, op.code, op.a, op.b, static_cast<unsigned long long>(op.imm));
+    s += line;
+  }
+}
+
+// One kernel for a program with its column and output types.  Short
+// programs unroll the row loops (more loads in flight per wave); long ones
+// keep one copy of each phase.
+std::string jit_source(const rfx_mf_program& p, const rfx_mf_col* cols, const rfx_mf_out* outs) {
+  std::string s =
+      "#include \"mapfilter_dev.h\"\n"
+      "using namespace rfx;\n"
+      "extern \"C\" __global__ __launch_bounds__(256) void rfx_mf_jit(\n"
+      "    MfIn in, MfOut out, uint64_t n, uint32_t ntiles, uint64_t* __restrict__ n_kept,\n"
+      "    uint64_t* __restrict__ status, uint32_t* __restrict__ ticket) {\n"
+      "  auto filter = [&](uint64_t row) -> bool {\n";
+  if (p.n_filter == 0) {
+    s += "    return true;\n";
+  } else {
+    s += "    uint64_t R[16] = {};\n";
+    jit_ops(s, p, 0, p.n_filter, cols, outs);
+    s += "    return R[" + std::to_string(p.keep_reg) + "] != 0;\n";
+  }
+  s += "  };\n  auto emit = [&](uint64_t row, uint64_t pos) {\n    uint64_t R[16] = {};\n";
+  jit_ops(s, p, p.n_filter, p.n_ops, cols, outs);
+  s += "  };\n  mapfilter_tile<" + std::to_string(p.n_ops <= 32 ? kMVT : 1) +
+       ">(filter, emit, n, ntiles, n_kept, status, ticket);\n}\n";
+  return s;
+}
+
+#define RFX_RTC(call)                                                                  \
+  do {                                                                                 \
+    hiprtcResult rfx_r_ = (call);                                                      \
+    if (rfx_r_ != HIPRTC_SUCCESS) {                                                    \
+      hiprtcDestroyProgram(&prog);                                                     \
+      return fail(RFX_EHIP, "rfx_mapfilter: %s: %s", #call, hiprtcGetErrorString(rfx_r_)); \
+    }                                                                                  \
+  } while (0)
+
+// The compiled kernel for `src` on the current device, compiling it on
+// first use.  Holds the cache lock while compiling, so concurrent first
+// calls compile once.
+int jit_function(const std::string& src, hipFunction_t* fn) {
+  int device = 0;
+  RFX_HIP(hipGetDevice(&device));
+  const std::string key = std::to_string(device) + "\n" + src;
+  std::lock_guard<std::mutex> lock(g_jit_mu);
+  auto it = g_jit_fns.find(key);
+  if (it != g_jit_fns.end()) {
+    *fn = it->second;
+    return RFX_OK;
+  }
+  constexpr int nh = sizeof(kJitHeaders) / sizeof(kJitHeaders[0]);
+  const char* names[nh];
+  const char* texts[nh];
+  for (int i = 0; i < nh; ++i) {
+    names[i] = kJitHeaders[i].name;
+    texts[i] = kJitHeaders[i].text;
+  }
+  hiprtcProgram prog = nullptr;
+  {
+    const hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "mapfilter_program.hip", nh,
+                                               texts, names);
+    if (r != HIPRTC_SUCCESS)
+      return fail(RFX_EHIP, "rfx_mapfilter: hiprtcCreateProgram: %s", hiprtcGetErrorString(r));
+  }
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult cr = hiprtcCompileProgram(prog, sizeof(opts) / sizeof(opts[0]), opts);
+  if (cr != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    std::string log;
+    if (hiprtcGetProgramLogSize(prog, &n) == HIPRTC_SUCCESS && n > 1) {
+      log.resize(n);
+      hiprtcGetProgramLog(prog, &log[0]);
+      if (log.size() > 400) log.resize(400);
+    }
+    hiprtcDestroyProgram(&prog);
+    return fail(RFX_EHIP, "rfx_mapfilter: compiling the program failed: %s", log.c_str());
+  }
+  size_t code_bytes = 0;
+  RFX_RTC(hiprtcGetCodeSize(prog, &code_bytes));
+  std::vector<char> code(code_bytes);
+  RFX_RTC(hiprtcGetCode(prog, code.data()));
+  hiprtcDestroyProgram(&prog);
+  hipModule_t mod = nullptr;
+  RFX_HIP(hipModuleLoadData(&mod, code.data()));
+  hipFunction_t f = nullptr;
+  const hipError_t e = hipModuleGetFunction(&f, mod, kJitName);
+  if (e != hipSuccess) {
+    (void)hipModuleUnload(mod);
+    return fail(RFX_EHIP, "rfx_mapfilter: hipModuleGetFunction: %s", hipGetErrorString(e));
+  }
+  g_jit_fns.emplace(key, f);
+  *fn = f;
+  return RFX_OK;
+}
+
+bool jit_cached(const std::string& src) {
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return false;
+  std::lock_guard<std::mutex> lock(g_jit_mu);
+  return g_jit_fns.count(std::to_string(device) + "\n" + src) != 0;
 }
 
 }  // namespace
@@ -352,6 +281,21 @@ extern "C" int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, u
     if (op.d >= RFX_MF_REGS || (op.code != RFX_MF_LDC && (op.a >= RFX_MF_REGS || op.b >= RFX_MF_REGS)))
       return fail(RFX_EINVAL, "rfx_mapfilter: op %u register out of range", pc);
   }
+  // Which kernel: mode 2 interprets; mode 1 must compile; the default
+  // compiles and interprets only if that fails, or while a graph is being
+  // captured and the program is not compiled yet (loading a module is not
+  // a stream operation a graph can hold).
+  hipFunction_t fn = nullptr;
+  if (ctx->tune_mf_mode != 2) {
+    const std::string src = jit_source(*prog, cols, outs);
+    if (ctx->capturing && !jit_cached(src)) {
+      if (ctx->tune_mf_mode == 1)
+        return fail(RFX_EINVAL, "rfx_mapfilter: cannot compile a program during graph capture");
+    } else {
+      const int s = jit_function(src, &fn);
+      if (s != RFX_OK && ctx->tune_mf_mode == 1) return s;
+    }
+  }
   if (n_rows == 0) return fill_u32(ctx, reinterpret_cast<uint32_t*>(n_kept), 2, 0);
   const uint64_t ntiles = (n_rows + kMTile - 1) / kMTile;
   Carve c(nullptr);
@@ -363,9 +307,15 @@ extern "C" int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, u
   uint32_t* ticket = w.take<uint32_t>(1);
   RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(status),
                    (reinterpret_cast<char*>(ticket + 1) - reinterpret_cast<char*>(status)) / 4, 0));
+  uint32_t nt = static_cast<uint32_t>(ntiles);
+  if (fn) {
+    RFX_PROF(ctx, "mapfilter");
+    void* args[] = {&in, &out, &n_rows, &nt, &n_kept, &status, &ticket};
+    RFX_HIP(hipModuleLaunchKernel(fn, nt, 1, 1, kMNT, 1, 1, 0, ctx->stream, args, nullptr));
+    return RFX_OK;
+  }
   RFX_PROF(ctx, "mapfilter");
-  k_mapfilter<<<static_cast<uint32_t>(ntiles), kMNT, 0, ctx->stream>>>(
-      in, n_rows, *prog, out, static_cast<uint32_t>(ntiles), n_kept, status, ticket);
+  k_mapfilter<<<nt, kMNT, 0, ctx->stream>>>(in, n_rows, *prog, out, nt, n_kept, status, ticket);
   RFX_LAUNCHED();
   return RFX_OK;
 }

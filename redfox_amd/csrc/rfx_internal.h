@@ -30,6 +30,7 @@ struct rfx_ctx {
   int tune_sort_ipt = 0;      // keys per thread of the onesweep tile
   int tune_fb_tile = 0;       // merged keys per find_bounds tile
   int tune_ex_tile = 0;       // merged items per expansion tile
+  int tune_mf_mode = 0;       // MAPFILTER: 0 compiled (else interpreted), 1 compiled, 2 interpreted
   std::vector<rfx_prof_rec> prof_pending;
   std::vector<hipEvent_t> prof_pool;
   std::map<std::string, std::pair<uint64_t, double>> prof_stats;  // launches, ms

@@ -1,7 +1,8 @@
 """MAPFILTER on the GPU against the oracle's row-by-row C-semantics
 evaluation: bit-exact integers and doubles (one IEEE rounding per
 operation, no fused multiply-add on either side), kept rows in input
-order."""
+order.  Every case runs twice: with the program compiled for its types,
+and through the interpreter."""
 import numpy as np
 import pytest
 
@@ -9,6 +10,13 @@ from redfox_amd import datagen
 from redfox_amd import mapfilter as M
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=[1, 2], ids=["compiled", "interpreted"])
+def mctx(ctx, request):
+    ctx.tune("mapfilter.mode", request.param)
+    yield ctx
+    ctx.tune("mapfilter.mode", 0)
 
 NP = {"u8": np.uint8, "u16": np.uint16, "u32": np.uint32, "u64": np.uint64, "i8": np.int8,
       "i16": np.int16, "i32": np.int32, "i64": np.int64, "f32": np.float32, "f64": np.float64}
@@ -39,7 +47,7 @@ def run(rf, ctx, orc, arrays, types, build, n=None):
     return kept, got
 
 
-def test_q1_projection(rf, ctx, orc):
+def test_q1_projection(rf, mctx, orc):
     """Q1's filter and computed columns (q1.man.cpp:959-2180): shipdate
     <= 1998-12-01 - 90 days, disc_price = price * (1 - disc), charge =
     disc_price * (1 + tax)."""
@@ -55,14 +63,14 @@ def test_q1_projection(rf, ctx, orc):
         x.emit(f * 2 + st, "u8")
         x.emit(r, "u32")
 
-    kept, got = run(rf, ctx, orc, [ship, flag, status, qty, price, disc, tax, rows],
+    kept, got = run(rf, mctx, orc, [ship, flag, status, qty, price, disc, tax, rows],
                     ["i32", "u8", "u8", "i32", "f64", "f64", "f64", "u32"], build)
     assert 0 < kept < n
     assert np.all(np.diff(got[4].astype(np.int64)) > 0)  # input order
 
 
 @pytest.mark.parametrize("n", [0, 1, 63, 2047, 2048, 2049, 100_000, (1 << 22) + 3])
-def test_sizes_and_order(rf, ctx, orc, n):
+def test_sizes_and_order(rf, mctx, orc, n):
     v = datagen.splitmix64(7 + n, n)
     rows = np.arange(n, dtype=np.uint64)
 
@@ -71,16 +79,16 @@ def test_sizes_and_order(rf, ctx, orc, n):
         x.emit(r, "u64")
         x.emit(a, "u64")
 
-    run(rf, ctx, orc, [v.view(np.int64), rows], ["i64", "u64"], build, n)
+    run(rf, mctx, orc, [v.view(np.int64), rows], ["i64", "u64"], build, n)
 
 
-def test_keep_all_and_none(rf, ctx, orc):
+def test_keep_all_and_none(rf, mctx, orc):
     v = np.arange(50_000, dtype=np.int32)
-    run(rf, ctx, orc, [v], ["i32"], lambda x, a: x.emit(a * 3, "i64"))
-    run(rf, ctx, orc, [v], ["i32"], lambda x, a: (x.where(a < -1), x.emit(a, "i32")))
+    run(rf, mctx, orc, [v], ["i32"], lambda x, a: x.emit(a * 3, "i64"))
+    run(rf, mctx, orc, [v], ["i32"], lambda x, a: (x.where(a < -1), x.emit(a, "i32")))
 
 
-def test_integer_edges(rf, ctx, orc):
+def test_integer_edges(rf, mctx, orc):
     """Truncating division and C remainder with zero and -1 divisors, the
     most negative value, wrapping products, signed against unsigned
     comparison of 64-bit columns."""
@@ -99,10 +107,10 @@ def test_integer_edges(rf, ctx, orc):
         x.emit((u_ < 0x8000000000000000) + 2 * (a_ < 0) + 4 * (u_ > a_), "u8")
         x.emit(~(a_ > b_) | (b_.eq(0)), "i16")
 
-    run(rf, ctx, orc, [a, b, u], ["i64", "i64", "u64"], build)
+    run(rf, mctx, orc, [a, b, u], ["i64", "i64", "u64"], build)
 
 
-def test_float_ops_and_conversions(rf, ctx, orc):
+def test_float_ops_and_conversions(rf, mctx, orc):
     n = 30_000
     f = (datagen.uniform01(21, n) - 0.5) * 1e6
     g = datagen.uniform01(22, n) - 0.25
@@ -119,10 +127,10 @@ def test_float_ops_and_conversions(rf, ctx, orc):
         x.emit(h_ + i_, "f64")
         x.emit(M.to_float(i_) / 7, "f64")
 
-    run(rf, ctx, orc, [f, g, i, h], ["f64", "f64", "i32", "f32"], build)
+    run(rf, mctx, orc, [f, g, i, h], ["f64", "f64", "i32", "f32"], build)
 
 
-def test_date_operations(rf, ctx, orc):
+def test_date_operations(rf, mctx, orc):
     """AddMonth (with Date.h's month wrap), AddYear and PartYear on dates
     in seconds since 1970 over 1992-1998, and a date-range filter."""
     n = 50_000
@@ -138,10 +146,10 @@ def test_date_operations(rf, ctx, orc):
         x.emit(M.part_year(s), "i32")
         x.emit(M.part_year(M.add_month(s, -1)), "i32")
 
-    run(rf, ctx, orc, [sec], ["i64"], build)
+    run(rf, mctx, orc, [sec], ["i64"], build)
 
 
-def test_packed_tuple_fields(rf, ctx, orc):
+def test_packed_tuple_fields(rf, mctx, orc):
     """Fields of 8- and 16-byte packed tuples: unsigned, signed and double,
     one field straddling the two words of a 16-byte tuple."""
     n = 40_000
@@ -162,10 +170,10 @@ def test_packed_tuple_fields(rf, ctx, orc):
         x.emit(mid, "u64")
         x.emit(d * 2.5, "f64")
 
-    run(rf, ctx, orc, [t8, t16], ["tuple8", "tuple16"], build)
+    run(rf, mctx, orc, [t8, t16], ["tuple8", "tuple16"], build)
 
 
-def test_many_wide_tuple_columns(rf, ctx, orc):
+def test_many_wide_tuple_columns(rf, mctx, orc):
     """Ten 16-byte tuple columns (160 bytes a row), a field of each."""
     n = 30_000
     cols = [np.stack([datagen.splitmix64(50 + 2 * c, n), datagen.splitmix64(51 + 2 * c, n)],
@@ -180,10 +188,10 @@ def test_many_wide_tuple_columns(rf, ctx, orc):
         x.emit(acc, "i64")
         x.emit(x.field(9, 0, 64, "unsigned"), "u64")
 
-    run(rf, ctx, orc, cols, ["tuple16"] * 10, build)
+    run(rf, mctx, orc, cols, ["tuple16"] * 10, build)
 
 
-def test_bad_programs_are_refused(rf, ctx):
+def test_bad_programs_are_refused(rf, mctx):
     a = rf.DeviceBuffer.from_numpy(np.arange(10, dtype=np.int32))
     x = M.MapFilter([(a, "i32")])
     x.emit(x.cols[0], "i32")
@@ -196,5 +204,75 @@ def test_bad_programs_are_refused(rf, ctx):
     outs = (M.Col * 1)()
     outs[0].data, outs[0].type = out.ptr, M.TYPES["i32"]
     cell = rf.u64_cell()
-    rc = rf.lib().rfx_mapfilter(ctx.handle, cols, 1, 10, ctypes.byref(prog), outs, 1, cell.ptr)
+    rc = rf.lib().rfx_mapfilter(mctx.handle, cols, 1, 10, ctypes.byref(prog), outs, 1, cell.ptr)
     assert rc == -1 and b"reads column 5" in rf.lib().rfx_last_error()
+
+
+def test_long_program(rf, mctx, orc):
+    """A program near the op limit (the compiled kernel keeps one copy of
+    each phase for programs this long)."""
+    n = 70_001
+    a = (datagen.splitmix64(61, n) % np.uint64(1 << 20)).astype(np.int64)
+    f = datagen.uniform01(62, n)
+
+    def build(x, a_, f_):
+        x.where(((a_ % 5) < 3) & (f_ < 0.9))
+        acc = a_
+        for k in range(1, 6):
+            acc = acc * (k + 1) - (a_ % (k + 2)) + k
+        x.emit(acc, "i64")
+        g = f_
+        for k in range(5):
+            g = g * 1.5 + M.to_float(a_ % (k + 3))
+        x.emit(g, "f64")
+
+    x = M.MapFilter([(rf.DeviceBuffer(8), "i64"), (rf.DeviceBuffer(8), "f64")])
+    build(x, *x.cols)
+    assert x.program().n_ops > 32
+    run(rf, mctx, orc, [a, f], ["i64", "f64"], build)
+
+
+@pytest.mark.parametrize("compile_first", [False, True], ids=["uncompiled", "compiled"])
+def test_graph_capture_and_replay(rf, orc, compile_first):
+    """Captured into a HIP graph on the context's own stream: a program not
+    yet compiled is recorded as the interpreter, a compiled one as its own
+    kernel.  Every replay (eager work and synchronisation between them)
+    rewrites the outputs and the kept count."""
+    n = 123_457
+    v = datagen.splitmix64(71 + int(compile_first), n).view(np.int64)
+    stream = rf.Stream()
+    c = rf.Context(0, stream.handle)
+    try:
+        col = rf.DeviceBuffer.from_numpy(v)
+        x = M.MapFilter([(col, "i64")])
+        a = x.cols[0]
+        salt = 1000 + 17 * int(compile_first)  # a program no other test compiled
+        x.where((a % 7) < 4)
+        x.emit(a % salt, "i64")
+        out = rf.DeviceBuffer(8 * n)
+        cell = rf.u64_cell()
+        x.run(c, n, [out], cell)  # sizes the workspace (and compiles, by default)
+        c.synchronize()
+        if not compile_first:
+            x = M.MapFilter([(col, "i64")])
+            a = x.cols[0]
+            x.where((a % 7) < 4)
+            x.emit(a % (salt + 1), "i64")
+        ft, ot = x.trees()
+        keep, want = orc.mapfilter(ft, ot, [v], n)
+        c.graph_begin()
+        x.run(c, n, [out], cell)
+        g = c.graph_end()
+        sentinel = np.array([7], dtype=np.uint64)
+        for _ in range(3):
+            out.zero()
+            cell.upload(sentinel)
+            assert rf.lib().rfx_device_synchronize() == 0
+            g.launch()
+            c.synchronize()
+            assert rf.read_u64(cell) == int(keep.sum())
+            assert np.array_equal(out.download(np.int64, int(keep.sum())), want[0])
+        g.close()
+    finally:
+        c.close()
+        stream.close()
