@@ -53,17 +53,21 @@ __device__ __forceinline__ uint64_t run_ops(const rfx_mf_program& prog, uint32_t
   return R[prog.keep_reg];
 }
 
-// The row loops stay rolled: each holds one copy of the interpreter.
+// The row loops stay rolled: each holds one copy of the interpreter.  The
+// output phase runs the output ops (which store) in place of compute.
 __global__ __launch_bounds__(kMNT) void k_mapfilter(MfIn in, uint64_t n, rfx_mf_program prog,
                                                     MfOut out, uint32_t ntiles,
                                                     uint64_t* __restrict__ n_kept,
                                                     uint64_t* __restrict__ status,
                                                     uint32_t* __restrict__ ticket) {
-  mapfilter_tile<1>(
+  mapfilter_tile<1, 1>(
       [&](uint64_t r) {
         return prog.n_filter == 0 || run_ops(prog, 0, prog.n_filter, in, out, r, 0) != 0;
       },
-      [&](uint64_t r, uint64_t pos) { run_ops(prog, prog.n_filter, prog.n_ops, in, out, r, pos); },
+      [](uint64_t) { return Empty{}; },
+      [&](uint64_t r, uint64_t pos, Empty) {
+        run_ops(prog, prog.n_filter, prog.n_ops, in, out, r, pos);
+      },
       n, ntiles, n_kept, status, ticket);
 }
 
@@ -111,9 +115,12 @@ std::unordered_map<std::string, hipFunction_t> g_jit_fns;
 
 constexpr const char kJitName[] = "rfx_mf_jit";
 
+// One statement per op.  In the output phase OUT ops copy their register
+// into the row's value slot (store_out happens in the store step).
 void jit_ops(std::string& s, const rfx_mf_program& p, uint32_t begin, uint32_t end,
-             const rfx_mf_col* cols, const rfx_mf_out* outs) {
+             const rfx_mf_col* cols) {
   char line[160];
+  uint32_t slot = 0;
   for (uint32_t pc = begin; pc < end; ++pc) {
     const rfx_mf_op& op = p.ops[pc];
     if (op.code == RFX_MF_NOP) continue;
@@ -121,8 +128,7 @@ void jit_ops(std::string& s, const rfx_mf_program& p, uint32_t begin, uint32_t e
       std::snprintf(line, sizeof line, "    R[%u] = load_elem(in.p[%u], %d, row, 0x%llxull);\n",
                     op.d, op.a, cols[op.a].type, static_cast<unsigned long long>(op.imm));
     else if (op.code == RFX_MF_OUT)
-      std::snprintf(line, sizeof line, "    store_out(out.p[%u], %d, pos, R[%u]);\n", op.b,
-                    outs[op.b].type, op.a);
+      std::snprintf(line, sizeof line, "    o.v[%u] = R[%u];\n", slot++, op.a);
     else
       std::snprintf(line, sizeof line, "    R[%u] = mf_apply(%u, R[%u], R[%u], 0x%llxull);\n",
                     op.d, op.code, op.a, op.b, static_cast<unsigned long long>(op.imm));
@@ -131,13 +137,19 @@ void jit_ops(std::string& s, const rfx_mf_program& p, uint32_t begin, uint32_t e
 }
 
 // One kernel for a program with its column and output types.  Short
-// programs unroll the row loops (more loads in flight per wave); long ones
-// keep one copy of each phase.
+// phases run their row batches unrolled (more loads in flight per wave);
+// long ones, or outputs with many values per row, keep fewer copies.
 std::string jit_source(const rfx_mf_program& p, const rfx_mf_col* cols, const rfx_mf_out* outs) {
+  uint32_t n_out_ops = 0;
+  for (uint32_t pc = p.n_filter; pc < p.n_ops; ++pc) n_out_ops += p.ops[pc].code == RFX_MF_OUT;
+  const uint32_t n_emit = p.n_ops - p.n_filter;
+  const int uf = p.n_filter <= 24 ? 8 : p.n_filter <= 48 ? 2 : 1;
+  const int uo = (n_emit <= 24 && n_out_ops <= 4) ? 8 : (n_emit <= 48 && n_out_ops <= 8) ? 2 : 1;
   std::string s =
       "#include \"mapfilter_dev.h\"\n"
       "using namespace rfx;\n"
-      "extern \"C\" __global__ __launch_bounds__(256) void rfx_mf_jit(\n"
+      "struct Vals { uint64_t v[" + std::to_string(n_out_ops ? n_out_ops : 1) + "]; };\n"
+      "extern \"C\" __global__ __launch_bounds__(kMNT) void rfx_mf_jit(\n"
       "    MfIn in, MfOut out, uint64_t n, uint32_t ntiles, uint64_t* __restrict__ n_kept,\n"
       "    uint64_t* __restrict__ status, uint32_t* __restrict__ ticket) {\n"
       "  auto filter = [&](uint64_t row) -> bool {\n";
@@ -145,13 +157,23 @@ std::string jit_source(const rfx_mf_program& p, const rfx_mf_col* cols, const rf
     s += "    return true;\n";
   } else {
     s += "    uint64_t R[16] = {};\n";
-    jit_ops(s, p, 0, p.n_filter, cols, outs);
+    jit_ops(s, p, 0, p.n_filter, cols);
     s += "    return R[" + std::to_string(p.keep_reg) + "] != 0;\n";
   }
-  s += "  };\n  auto emit = [&](uint64_t row, uint64_t pos) {\n    uint64_t R[16] = {};\n";
-  jit_ops(s, p, p.n_filter, p.n_ops, cols, outs);
-  s += "  };\n  mapfilter_tile<" + std::to_string(p.n_ops <= 32 ? kMVT : 1) +
-       ">(filter, emit, n, ntiles, n_kept, status, ticket);\n}\n";
+  s += "  };\n  auto compute = [&](uint64_t row) -> Vals {\n    Vals o{};\n    uint64_t R[16] = {};\n";
+  jit_ops(s, p, p.n_filter, p.n_ops, cols);
+  s += "    return o;\n  };\n  auto store = [&](uint64_t, uint64_t pos, const Vals& o) {\n";
+  char line[128];
+  uint32_t slot = 0;
+  for (uint32_t pc = p.n_filter; pc < p.n_ops; ++pc) {
+    const rfx_mf_op& op = p.ops[pc];
+    if (op.code != RFX_MF_OUT) continue;
+    std::snprintf(line, sizeof line, "    store_out(out.p[%u], %d, pos, o.v[%u]);\n", op.b,
+                  outs[op.b].type, slot++);
+    s += line;
+  }
+  s += "  };\n  mapfilter_tile<" + std::to_string(uf) + ", " + std::to_string(uo) +
+       ">(filter, compute, store, n, ntiles, n_kept, status, ticket);\n}\n";
   return s;
 }
 

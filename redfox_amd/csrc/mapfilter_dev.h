@@ -17,9 +17,9 @@
 
 namespace rfx {
 
-constexpr int kMNT = 256;
+constexpr int kMNT = 512;
 constexpr int kMNW = kMNT / kWave;
-constexpr int kMVT = 8;
+constexpr int kMVT = 16;
 constexpr int kMTile = kMNT * kMVT;
 
 __device__ __forceinline__ double as_f(uint64_t v) {
@@ -203,18 +203,28 @@ struct MfOut {
   int32_t t[RFX_MF_MAX_OUTS];
 };
 
-// One tile of kMTile rows.  filter(r) says whether row r is kept;
-// emit(r, pos) stores row r's outputs at position pos.  Each wave ballots
-// the keep flags of 64 consecutive rows per step, the (step, wave) counts
-// are scanned in LDS, the tile's offset comes from a decoupled look-back,
-// and the kept rows are then emitted in input order.  UNROLL is the unroll
-// count of the row loops (1 keeps a large filter/emit body to one copy).
-template <int UNROLL, class Filter, class Emit>
-__device__ __forceinline__ void mapfilter_tile(Filter filter, Emit emit, uint64_t n,
-                                               uint32_t ntiles, uint64_t* n_kept,
+// One tile of kMTile rows, in two phases.
+//
+// Filter: filter(r) says whether row r is kept.  It runs on every row in
+// batches of UF steps of kMNT rows, rows past the end reading the last row
+// (so a batch is straight-line code whose loads are all in flight at
+// once) and counting as dropped.  Each wave ballots the flags of its 64
+// rows per step, the (step, wave) counts are scanned in LDS, the kept rows'
+// tile offsets are compacted into LDS, and the tile's output offset comes
+// from a decoupled look-back.
+//
+// Outputs: compute(r) returns row r's values, store(r, pos, v) writes them
+// at position pos.  Thread t of step j takes the (kMNT j + t)-th kept row, so
+// the phase reads kept rows only, every step but the tile's last is dense,
+// and stores go to consecutive positions.  A batch computes UO rows (the
+// last kept row standing in past the tile's count) before it stores any.
+template <int UF, int UO, class Filter, class Compute, class Store>
+__device__ __forceinline__ void mapfilter_tile(Filter filter, Compute compute, Store store,
+                                               uint64_t n, uint32_t ntiles, uint64_t* n_kept,
                                                uint64_t* status, uint32_t* ticket) {
+  static_assert(kMVT % UF == 0 && kMVT % UO == 0, "batches divide the tile");
   __shared__ uint32_t s_cnt[kMVT * kMNW + 1];
-  __shared__ uint64_t s_ball[kMVT * kMNW];
+  __shared__ uint16_t s_rows[kMTile];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
@@ -222,50 +232,83 @@ __device__ __forceinline__ void mapfilter_tile(Filter filter, Emit emit, uint64_
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint64_t base = static_cast<uint64_t>(tile) * kMTile;
-  // Step j of wave w covers rows base + j*256 + w*64 .. +63.
   uint32_t keep = 0;
-#pragma unroll UNROLL
-  for (int j = 0; j < kMVT; ++j) {
-    const uint64_t r = base + static_cast<uint64_t>(j) * kMNT + tid;
-    const bool k = r < n && filter(r);
-    keep |= static_cast<uint32_t>(k) << j;
-    const uint64_t m = ballot(k);
-    if (lane == 0) {
-      s_cnt[j * kMNW + w] = static_cast<uint32_t>(__popcll(m));
-      s_ball[j * kMNW + w] = m;
+#pragma unroll 1
+  for (int j0 = 0; j0 < kMVT; j0 += UF) {
+#pragma unroll
+    for (int u = 0; u < UF; ++u) {
+      const uint64_t r = base + static_cast<uint64_t>(j0 + u) * kMNT + tid;
+      keep |= static_cast<uint32_t>(filter(r < n ? r : n - 1) & (r < n)) << (j0 + u);
     }
+  }
+  uint64_t ball[kMVT];
+#pragma unroll
+  for (int j = 0; j < kMVT; ++j) ball[j] = ballot((keep >> j) & 1u);
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < kMVT; ++j) s_cnt[j * kMNW + w] = static_cast<uint32_t>(__popcll(ball[j]));
   }
   __syncthreads();
   if (w == 0) {
-    const uint32_t c = lane < kMVT * kMNW ? s_cnt[lane] : 0u;
-    const uint32_t incl = wave_incl_scan(c);
-    if (lane < kMVT * kMNW) s_cnt[lane] = incl - c;
-    const uint32_t tile_kept = __shfl(incl, kWave - 1, kWave);
-    if (lane == 0) s_cnt[kMVT * kMNW] = tile_kept;
+    // Lane l scans entries [l*P, l*P + P) (entry j*kMNW + w: row order).
+    constexpr int kE = kMVT * kMNW, P = kE > kWave ? kE / kWave : 1;
+    static_assert(kE <= kWave || kE % kWave == 0, "whole runs per lane");
+    uint32_t c[P], sum = 0;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const uint32_t e = lane * P + q;
+      c[q] = e < kE ? s_cnt[e] : 0u;
+      sum += c[q];
+    }
+    const uint32_t incl = wave_incl_scan(sum);
+    uint32_t run = incl - sum;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const uint32_t e = lane * P + q;
+      if (e < kE) s_cnt[e] = run;
+      run += c[q];
+    }
+    if (lane == kWave - 1) s_cnt[kE] = incl;
+  }
+  __syncthreads();
+  const uint32_t kept = s_cnt[kMVT * kMNW];
+#pragma unroll
+  for (int j = 0; j < kMVT; ++j) {
+    if ((ball[j] >> lane) & 1u)
+      s_rows[s_cnt[j * kMNW + w] + mbcnt(ball[j])] = static_cast<uint16_t>(j * kMNT + tid);
+  }
+  if (w == 0) {
     if (tile == 0) {
       if (lane == 0) {
-        status_store64(status, kFlagIncl64 | tile_kept);
+        status_store64(status, kFlagIncl64 | kept);
         s_prefix = 0;
       }
     } else {
-      if (lane == 0) status_store64(status + tile, kFlagAgg64 | tile_kept);
+      if (lane == 0) status_store64(status + tile, kFlagAgg64 | kept);
       const uint64_t excl = wave_lookback64(status, tile);
       if (lane == 0) {
-        status_store64(status + tile, kFlagIncl64 | (excl + tile_kept));
+        status_store64(status + tile, kFlagIncl64 | (excl + kept));
         s_prefix = excl;
       }
     }
   }
   __syncthreads();
   const uint64_t prefix = s_prefix;
-#pragma unroll UNROLL
-  for (int j = 0; j < kMVT; ++j) {
-    if ((keep >> j) & 1u) {
-      const uint64_t r = base + static_cast<uint64_t>(j) * kMNT + tid;
-      emit(r, prefix + s_cnt[j * kMNW + w] + mbcnt(s_ball[j * kMNW + w]));
+  if (tile == ntiles - 1 && tid == 0) *n_kept = prefix + kept;
+#pragma unroll 1
+  for (uint32_t i0 = 0; i0 < kept; i0 += UO * kMNT) {
+    decltype(compute(base)) v[UO];
+#pragma unroll
+    for (int u = 0; u < UO; ++u) {
+      const uint32_t i = i0 + u * kMNT + tid;
+      v[u] = compute(base + s_rows[i < kept ? i : kept - 1]);
+    }
+#pragma unroll
+    for (int u = 0; u < UO; ++u) {
+      const uint32_t i = i0 + u * kMNT + tid;
+      if (i < kept) store(base + s_rows[i], prefix + i, v[u]);
     }
   }
-  if (tile == ntiles - 1 && tid == 0) *n_kept = prefix + s_cnt[kMVT * kMNW];
 }
 
 }  // namespace rfx

@@ -69,7 +69,7 @@ def test_q1_projection(rf, mctx, orc):
     assert np.all(np.diff(got[4].astype(np.int64)) > 0)  # input order
 
 
-@pytest.mark.parametrize("n", [0, 1, 63, 2047, 2048, 2049, 100_000, (1 << 22) + 3])
+@pytest.mark.parametrize("n", [0, 1, 63, 511, 512, 8191, 8192, 8193, 100_000, (1 << 22) + 3])
 def test_sizes_and_order(rf, mctx, orc, n):
     v = datagen.splitmix64(7 + n, n)
     rows = np.arange(n, dtype=np.uint64)
