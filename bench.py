@@ -581,6 +581,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ops", action="store_true", help="headline sort only")
+    ap.add_argument("--only", default="", help="comma-separated operator lines to run "
+                    "besides the headline sort (e.g. mapfilter_2^27)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not supported "
@@ -599,27 +601,26 @@ def main():
     log(f"  {s['ms_per_step']:.3f} ms/step, {s['rows_per_s'] / 1e9:.2f} G rows/s, "
         f"onesweep {s['roofline']['launch_ms']:.3f} ms/launch frac {s['roofline']['frac']}")
     ops = {"sort_u32_2^28": s}
+    lines = [
+        ("sort_u64_zipf_2^27", "sort 2^27 u64 Zipf(1.1) pairs", lambda: bench_sort_u64_zipf(rf, ctx)),
+        ("join_65536x262144", "join config 1", lambda: bench_join(rf, ctx, 65536, 262144, 16, 20)),
+        ("join_2^24x2^26", "join 2^24 x 2^26", lambda: bench_join(rf, ctx, 1 << 24, 1 << 26, 24, 5)),
+        ("q1_60M", "q1 60M", lambda: bench_q1(rf, ctx)),
+        ("agg_2^28", "group-by sum+count 2^28", lambda: bench_agg(rf, ctx)),
+        ("select_2^27", "select 2^27 16-byte tuples", lambda: bench_select(rf, ctx)),
+        ("mapfilter_2^27", "mapfilter 2^27 lineitem rows", lambda: bench_mapfilter(rf, ctx)),
+        ("setop_2^27", "set operations 2^27 + 2^27 u64", lambda: bench_setop(rf, ctx)),
+        ("batched_512", "batched 512 join-aggregate instances", lambda: bench_batched(rf, ctx)),
+        ("served_mix", "served mixed requests", lambda: bench_served(rf, ctx)),
+    ]
+    only = set(args.only.split(",")) if args.only else None
+    if only and not only <= {name for name, _, _ in lines}:
+        raise SystemExit(f"bench.py: unknown --only lines {sorted(only - {n for n, _, _ in lines})}")
     if not args.no_ops:
-        log("sort 2^27 u64 Zipf(1.1) pairs ...")
-        ops["sort_u64_zipf_2^27"] = bench_sort_u64_zipf(rf, ctx)
-        log("join config 1 ...")
-        ops["join_65536x262144"] = bench_join(rf, ctx, 65536, 262144, 16, 20)
-        log("join 2^24 x 2^26 ...")
-        ops["join_2^24x2^26"] = bench_join(rf, ctx, 1 << 24, 1 << 26, 24, 5)
-        log("q1 60M ...")
-        ops["q1_60M"] = bench_q1(rf, ctx)
-        log("group-by sum+count 2^28 ...")
-        ops["agg_2^28"] = bench_agg(rf, ctx)
-        log("select 2^27 16-byte tuples ...")
-        ops["select_2^27"] = bench_select(rf, ctx)
-        log("mapfilter 2^27 lineitem rows ...")
-        ops["mapfilter_2^27"] = bench_mapfilter(rf, ctx)
-        log("set operations 2^27 + 2^27 u64 ...")
-        ops["setop_2^27"] = bench_setop(rf, ctx)
-        log("batched 512 join-aggregate instances ...")
-        ops["batched_512"] = bench_batched(rf, ctx)
-        log("served mixed requests ...")
-        ops["served_mix"] = bench_served(rf, ctx)
+        for name, what, fn in lines:
+            if only is None or name in only:
+                log(what + " ...")
+                ops[name] = fn()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
