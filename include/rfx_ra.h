@@ -522,7 +522,7 @@ int rfx_harmony_call(const char* kernel_name, const uint64_t* args,
 /* ---------------------------------------------------------------------- */
 /* Replaces the per-instance generated arithmetic and selection kernels
  * (set_arith / set_convert, RelationalAlgebraKernel.cpp:2001-2215;
- * set_select, :2301-2593) with one interpreted register program: ops
+ * set_select, :2301-2593) with one register program: ops
  * [0, n_filter) run on every row and register keep_reg decides whether it
  * is kept (n_filter == 0 keeps every row); ops [n_filter, n_ops) run on the
  * kept rows, starting from cleared registers, and their OUT ops store to
@@ -530,7 +530,12 @@ int rfx_harmony_call(const char* kernel_name, const uint64_t* args,
  * *n_kept (device) receives the kept count.  Registers hold 64 bits: an
  * int64 (two's complement; C division and remainder, a zero divisor
  * gives 0) or the bits of a double (IEEE, no fused multiply-add).  The
- * date ops follow Date.h (seconds since 1970, TPC-H years 1992-1998). */
+ * date ops follow Date.h (seconds since 1970, TPC-H years 1992-1998).
+ * A program runs as a kernel compiled for it and its column and output
+ * types (hiprtc, on its first call; kept for the process), or through the
+ * interpreter kernel while a graph is being captured and the program is
+ * not compiled yet, or when compiling fails; "mapfilter.mode" in
+ * rfx_ctx_tune forces one.  Both give the same results. */
 #define RFX_MF_MAX_COLS 16
 #define RFX_MF_MAX_OUTS 8
 #define RFX_MF_MAX_OPS 96
