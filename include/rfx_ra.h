@@ -566,6 +566,8 @@ int rfx_harmony_call(const char* kernel_name, const uint64_t* args,
                               16-17 kind (0 unsigned, 1 signed, 2 double) */
 #define RFX_MF_LDI 2       /* R[d] = imm */
 #define RFX_MF_MOV 3       /* R[d] = R[a] */
+#define RFX_MF_LDX 4       /* R[d] = column a at row R[b] (a gather; imm as for LDC);
+                              a row at or past the column's length reads 0 */
 #define RFX_MF_IADD 10
 #define RFX_MF_ISUB 11
 #define RFX_MF_IMUL 12
@@ -611,6 +613,7 @@ typedef struct rfx_mf_program {
 typedef struct rfx_mf_col {
   const void* data;
   int32_t type, reserved;
+  uint64_t rows;  /* rows LDX may read (0: the pass's n_rows) */
 } rfx_mf_col;
 
 typedef struct rfx_mf_out {

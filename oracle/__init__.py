@@ -311,6 +311,22 @@ def mapfilter_eval(tree, cols):
                 m = np.uint64(1 << (bits - 1))
                 v = (v ^ m) - m
             return v.view(np.int64), ("u" if bits == 64 and kind == "unsigned" else "i")
+        if tag == "gather":
+            # Column tree[1] at the rows tree[2] gives; past its end, 0.
+            rows, _ = mapfilter_eval(tree[2], cols)
+            c = cols[tree[1]]
+            r = np.atleast_1d(np.asarray(rows, dtype=np.int64)).view(np.uint64)
+            inside = r < np.uint64(len(c))
+            picked = np.zeros((len(r),) + c.shape[1:], dtype=c.dtype)
+            picked[inside] = c[r[inside].astype(np.int64)]
+            if len(tree) > 3:
+                sub = list(cols)
+                sub[tree[1]] = picked
+                v, k = mapfilter_eval(("field", tree[1]) + tuple(tree[3]), sub)
+                return np.where(inside, v, 0), k
+            sub = list(cols)
+            sub[tree[1]] = picked
+            return mapfilter_eval(("col", tree[1]), sub)
         if tag == "int":
             return np.array(tree[1] & 0xFFFFFFFFFFFFFFFF, dtype=np.uint64).view(np.int64), "i"
         if tag == "float":

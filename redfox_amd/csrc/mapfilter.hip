@@ -44,6 +44,8 @@ __device__ __forceinline__ uint64_t run_ops(const rfx_mf_program& prog, uint32_t
     const rfx_mf_op op = prog.ops[pc];
     if (op.code == RFX_MF_LDC) {
       R[op.d] = load_elem(in.p[op.a], in.t[op.a], r, op.imm);
+    } else if (op.code == RFX_MF_LDX) {
+      R[op.d] = load_at(in.p[op.a], in.t[op.a], in.rows[op.a], R[op.b], op.imm);
     } else if (op.code == RFX_MF_OUT) {
       store_out(out.p[op.b], out.t[op.b], pos, R[op.a]);
     } else if (op.code != RFX_MF_NOP) {
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(kMNT) void k_mapfilter(MfIn in, uint64_t n, rfx_mf_
 bool valid_col_type(int t) { return t >= RFX_MF_U8 && t <= RFX_MF_TUPLE16; }
 
 bool valid_code(uint32_t c) {
-  return c <= RFX_MF_MOV || (c >= RFX_MF_IADD && c <= RFX_MF_IMOD) ||
+  return c <= RFX_MF_LDX || (c >= RFX_MF_IADD && c <= RFX_MF_IMOD) ||
          (c >= RFX_MF_FADD && c <= RFX_MF_FDIV) || (c >= RFX_MF_I2F && c <= RFX_MF_F2I) ||
          (c >= RFX_MF_ILT && c <= RFX_MF_FNE) || (c >= RFX_MF_AND && c <= RFX_MF_NOT) ||
          (c >= RFX_MF_ADD_MONTH && c <= RFX_MF_PART_YEAR) || c == RFX_MF_OUT;
@@ -127,6 +129,10 @@ void jit_ops(std::string& s, const rfx_mf_program& p, uint32_t begin, uint32_t e
     if (op.code == RFX_MF_LDC)
       std::snprintf(line, sizeof line, "    R[%u] = load_elem(in.p[%u], %d, row, 0x%llxull);\n",
                     op.d, op.a, cols[op.a].type, static_cast<unsigned long long>(op.imm));
+    else if (op.code == RFX_MF_LDX)
+      std::snprintf(line, sizeof line,
+                    "    R[%u] = load_at(in.p[%u], %d, in.rows[%u], R[%u], 0x%llxull);\n", op.d,
+                    op.a, cols[op.a].type, op.a, op.b, static_cast<unsigned long long>(op.imm));
     else if (op.code == RFX_MF_OUT)
       std::snprintf(line, sizeof line, "    o.v[%u] = R[%u];\n", slot++, op.a);
     else
@@ -274,6 +280,10 @@ extern "C" int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, u
       return fail(RFX_EINVAL, "rfx_mapfilter: column %d (type %d)", c, cols[c].type);
     in.p[c] = cols[c].data;
     in.t[c] = cols[c].type;
+    in.rows[c] = cols[c].rows ? cols[c].rows : n_rows;
+    if (in.rows[c] && !cols[c].data)
+      return fail(RFX_EINVAL, "rfx_mapfilter: column %d has %llu rows and no data", c,
+                  static_cast<unsigned long long>(in.rows[c]));
   }
   for (int k = 0; k < n_outs; ++k) {
     if (outs[k].type < RFX_MF_U8 || outs[k].type > RFX_MF_F64 || (n_rows && !outs[k].data))
@@ -285,7 +295,7 @@ extern "C" int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, u
   for (uint32_t pc = 0; pc < prog->n_ops; ++pc) {
     const rfx_mf_op& op = prog->ops[pc];
     const bool filter = pc < prog->n_filter;
-    if (op.code == RFX_MF_LDC) {
+    if (op.code == RFX_MF_LDC || op.code == RFX_MF_LDX) {
       if (op.a >= n_cols) return fail(RFX_EINVAL, "rfx_mapfilter: op %u reads column %u", pc, op.a);
       if (cols[op.a].type >= RFX_MF_TUPLE1) {
         const uint32_t shift = op.imm & 0xFF, bits = (op.imm >> 8) & 0xFF;
@@ -300,7 +310,8 @@ extern "C" int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, u
     } else if (!valid_code(op.code)) {
       return fail(RFX_EINVAL, "rfx_mapfilter: op %u code %u", pc, op.code);
     }
-    if (op.d >= RFX_MF_REGS || (op.code != RFX_MF_LDC && (op.a >= RFX_MF_REGS || op.b >= RFX_MF_REGS)))
+    if (op.d >= RFX_MF_REGS || op.b >= RFX_MF_REGS ||
+        (op.code != RFX_MF_LDC && op.code != RFX_MF_LDX && op.a >= RFX_MF_REGS))
       return fail(RFX_EINVAL, "rfx_mapfilter: op %u register out of range", pc);
   }
   // Which kernel: mode 2 interprets; mode 1 must compile; the default

@@ -64,6 +64,12 @@ __device__ __forceinline__ uint64_t load_elem(const void* p, int t, uint64_t i, 
   return f;  // unsigned, or the bits of a double when kind == 2
 }
 
+// LDX: element i of a column of `rows` rows, or 0 past its end.
+__device__ __forceinline__ uint64_t load_at(const void* p, int t, uint64_t rows, uint64_t i,
+                                            uint64_t imm) {
+  return i < rows ? load_elem(p, t, i, imm) : 0;
+}
+
 // Output element: integers keep their low bits, F32 rounds the double.
 __device__ __forceinline__ void store_out(void* p, int t, uint64_t pos, uint64_t v) {
   switch (t) {
@@ -197,6 +203,7 @@ __device__ __forceinline__ uint64_t mf_apply(uint32_t code, uint64_t a, uint64_t
 struct MfIn {
   const void* p[RFX_MF_MAX_COLS];
   int32_t t[RFX_MF_MAX_COLS];
+  uint64_t rows[RFX_MF_MAX_COLS];  // bound of LDX's row index
 };
 struct MfOut {
   void* p[RFX_MF_MAX_OUTS];

@@ -24,7 +24,7 @@ from . import rfx
 TYPES = {"u8": 1, "u16": 2, "u32": 3, "u64": 4, "i8": 5, "i16": 6, "i32": 7, "i64": 8,
          "f32": 9, "f64": 10, "tuple1": 11, "tuple2": 12, "tuple4": 13, "tuple8": 14,
          "tuple16": 15}
-OP = dict(NOP=0, LDC=1, LDI=2, MOV=3, IADD=10, ISUB=11, IMUL=12, IDIV=13, IMOD=14, FADD=20,
+OP = dict(NOP=0, LDC=1, LDI=2, MOV=3, LDX=4, IADD=10, ISUB=11, IMUL=12, IDIV=13, IMOD=14, FADD=20,
           FSUB=21, FMUL=22, FDIV=23, I2F=30, U2F=31, F2I=32, ILT=40, ILE=41, IEQ=42, INE=43,
           ULT=44, ULE=45, FLT=46, FLE=47, FEQ=48, FNE=49, AND=50, OR=51, NOT=52, ADD_MONTH=60,
           ADD_YEAR=61, PART_YEAR=62, OUT=70)
@@ -43,7 +43,8 @@ class Program(ctypes.Structure):
 
 
 class Col(ctypes.Structure):
-    _fields_ = [("data", ctypes.c_void_p), ("type", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+    _fields_ = [("data", ctypes.c_void_p), ("type", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("rows", ctypes.c_uint64)]
 
 
 def _f64_bits(v: float) -> int:
@@ -93,6 +94,21 @@ class Column(Expr):
 
     def tree(self):
         return ("col", self.index) if self.field is None else ("field", self.index) + tuple(self.field)
+
+
+class Gather(Column):
+    """Column `index` at the row an integer expression gives (LDX); a row
+    past the column's length reads 0."""
+
+    def __init__(self, index: int, ctype: str, row: "Expr",
+                 field: Optional[Tuple[int, int, str]] = None):
+        super().__init__(index, ctype, field)
+        if row.type == "f":
+            raise ValueError("a gather's row must be an integer expression")
+        self.row = row
+
+    def tree(self):
+        return ("gather", self.index, self.row.tree()) + ((tuple(self.field),) if self.field else ())
 
 
 class Const(Expr):
@@ -191,7 +207,11 @@ class _Gen:
             if e.field is not None:
                 shift, bits, kind = e.field
                 imm = shift | (bits << 8) | ({"unsigned": 0, "signed": 1, "double": 2}[kind] << 16)
-            self.emit("LDC", r, e.index, 0, imm)
+            if isinstance(e, Gather):
+                self.gen(e.row, r)
+                self.emit("LDX", r, e.index, r, imm)
+            else:
+                self.emit("LDC", r, e.index, 0, imm)
         elif isinstance(e, Const):
             self.emit("LDI", r, 0, 0, _f64_bits(e.v) if e.type == "f" else int(e.v))
         elif isinstance(e, Bin):
@@ -240,16 +260,23 @@ class _Gen:
 class MapFilter:
     """A MAPFILTER over device columns: a filter and output expressions."""
 
-    def __init__(self, columns: Sequence[Tuple[object, str]]):
+    def __init__(self, columns: Sequence[Tuple]):
+        """columns: (buffer, type) or (buffer, type, rows); rows bounds the
+        rows a gather may read (default: the pass's row count)."""
         if len(columns) > MAX_COLS:
             raise ValueError("at most 16 input columns")
-        self.columns = list(columns)
-        self.cols = [Column(i, t) for i, (_, t) in enumerate(columns)]
+        self.columns = [tuple(c[:2]) for c in columns]
+        self.rows = [int(c[2]) if len(c) > 2 else 0 for c in columns]
+        self.cols = [Column(i, t) for i, (_, t) in enumerate(self.columns)]
         self.filter: Optional[Expr] = None
         self.outputs: List[Tuple[Expr, str]] = []
 
     def field(self, index: int, shift: int, bits: int, kind: str = "unsigned") -> Column:
         return Column(index, self.columns[index][1], (shift, bits, kind))
+
+    def gather(self, index: int, row, field: Optional[Tuple[int, int, str]] = None) -> Column:
+        """Column `index` at row `row` (an integer expression)."""
+        return Gather(index, self.columns[index][1], lift(row), field)
 
     def where(self, cond: Expr) -> "MapFilter":
         self.filter = cond
@@ -287,7 +314,7 @@ class MapFilter:
         prog = self.program()
         cols = (Col * max(len(self.columns), 1))()
         for i, (buf, t) in enumerate(self.columns):
-            cols[i].data, cols[i].type = buf.ptr, TYPES[t]
+            cols[i].data, cols[i].type, cols[i].rows = buf.ptr, TYPES[t], self.rows[i]
         outa = (Col * max(len(outs), 1))()
         for k, (buf, (_, t)) in enumerate(zip(outs, self.outputs)):
             outa[k].data, outa[k].type = buf.ptr, TYPES[t]

@@ -178,3 +178,20 @@ def test_abi_validates_before_touching_the_device():
     prog.ops[0].d = 0
     assert L.rfx_mapfilter(fake_ctx, cols, 1, 10, ctypes.byref(prog), None, 0, cell) == -1
     assert L.rfx_mapfilter(fake_ctx, cols, 17, 10, ctypes.byref(prog), None, 0, cell) == -1
+
+
+def test_gather_known_answers():
+    """The oracle's gather: rows past either end read 0; a 16-byte tuple
+    field straddling the words ((lo >> 60) | (hi << 4), 8 bits)."""
+    c = np.array([5, 6, 7], dtype=np.int32)
+    idx = np.array([2, 0, 3, -1, 1], dtype=np.int32)
+    v, k = oracle.mapfilter_eval(("gather", 1, ("col", 0)), [idx, c])
+    assert v.tolist() == [7, 5, 0, 0, 6] and k == "i"
+    t = np.array([[1 << 61, 3], [7, 1]], dtype=np.uint64)
+    v, _ = oracle.mapfilter_eval(("gather", 1, ("col", 0), (60, 8, "unsigned")),
+                                 [np.array([0, 1, 2]), t])
+    assert v.tolist() == [50, 16, 0]
+    x = M.MapFilter([(_Buf(), "i32"), (_Buf(), "f64", 10)])
+    x.emit(x.gather(1, x.cols[0] + 1), "f64")
+    p = x.program()
+    assert [p.ops[i].code for i in range(p.n_ops)][-2:] == [M.OP["LDX"], M.OP["OUT"]]

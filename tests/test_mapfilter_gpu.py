@@ -276,3 +276,35 @@ def test_graph_capture_and_replay(rf, orc, compile_first):
     finally:
         c.close()
         stream.close()
+
+
+def test_gather_rows(rf, mctx, orc):
+    """LDX: columns of another length read at computed rows (the join's
+    output gather), rows past the end reading 0, a packed tuple field read
+    through a gather, and arithmetic on gathered values."""
+    n, m = 60_001, 5_000
+    idx = (datagen.splitmix64(81, n) % np.uint64(m + 50)).astype(np.int32)  # some past the end
+    idx[:4] = [0, m - 1, m, -1]
+    vals = datagen.uniform01(82, m) * 100.0
+    small = (datagen.splitmix64(83, m) % np.uint64(200)).astype(np.uint8)
+    t16 = np.stack([datagen.splitmix64(84, m), datagen.splitmix64(85, m)], axis=1).copy()
+    bufs = [rf.DeviceBuffer.from_numpy(a) for a in (idx, vals, small, t16)]
+    x = M.MapFilter([(bufs[0], "i32"), (bufs[1], "f64", m), (bufs[2], "u8", m),
+                     (bufs[3], "tuple16", m)])
+    i = x.cols[0]
+    x.where(x.gather(2, i) < 150)
+    x.emit(x.gather(1, i) * 2.0 + x.gather(2, i + 1), "f64")
+    x.emit(x.gather(3, i, (60, 20, "unsigned")), "u32")
+    x.emit(x.gather(2, i), "u8")
+    outs = [rf.DeviceBuffer(8 * n), rf.DeviceBuffer(4 * n), rf.DeviceBuffer(n)]
+    cell = rf.u64_cell()
+    x.run(mctx, n, outs, cell)
+    mctx.synchronize()
+    kept = rf.read_u64(cell)
+    ft, ot = x.trees()
+    keep, want = orc.mapfilter(ft, ot, [idx, vals, small, t16], n)
+    assert kept == int(keep.sum()) and 0 < kept < n
+    got = [outs[0].download(np.float64, kept), outs[1].download(np.uint32, kept),
+           outs[2].download(np.uint8, kept)]
+    assert np.array_equal(got[0].view(np.uint64), want[0].view(np.uint64))
+    assert np.array_equal(got[1], want[1]) and np.array_equal(got[2], want[2])
