@@ -29,6 +29,11 @@
 #include "mapfilter_dev.h"
 #include "rfx_internal.h"
 
+// The layouts the Python binding (redfox_amd/mapfilter.py) mirrors.
+static_assert(sizeof(rfx_mf_op) == 16 && sizeof(rfx_mf_col) == 24 && sizeof(rfx_mf_out) == 16 &&
+                  sizeof(rfx_mf_program) == 16 + 16 * RFX_MF_MAX_OPS,
+              "MAPFILTER ABI structs");
+
 namespace rfx {
 namespace {
 

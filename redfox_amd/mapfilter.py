@@ -47,6 +47,10 @@ class Col(ctypes.Structure):
                 ("rows", ctypes.c_uint64)]
 
 
+class Out(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("type", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 def _f64_bits(v: float) -> int:
     return ctypes.c_uint64.from_buffer_copy(ctypes.c_double(float(v))).value
 
@@ -315,7 +319,7 @@ class MapFilter:
         cols = (Col * max(len(self.columns), 1))()
         for i, (buf, t) in enumerate(self.columns):
             cols[i].data, cols[i].type, cols[i].rows = buf.ptr, TYPES[t], self.rows[i]
-        outa = (Col * max(len(outs), 1))()
+        outa = (Out * max(len(outs), 1))()
         for k, (buf, (_, t)) in enumerate(zip(outs, self.outputs)):
             outa[k].data, outa[k].type = buf.ptr, TYPES[t]
         rfx._check("rfx_mapfilter", rfx.lib().rfx_mapfilter(

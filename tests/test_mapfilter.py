@@ -195,3 +195,12 @@ def test_gather_known_answers():
     x.emit(x.gather(1, x.cols[0] + 1), "f64")
     p = x.program()
     assert [p.ops[i].code for i in range(p.n_ops)][-2:] == [M.OP["LDX"], M.OP["OUT"]]
+
+
+def test_struct_layouts_match_the_header():
+    """rfx_mf_op 16 B, rfx_mf_col 24 B (data, type, reserved, rows),
+    rfx_mf_out 16 B, rfx_mf_program 16 + 16 * 96 B (static_assert in
+    mapfilter.hip)."""
+    assert ctypes.sizeof(M.Op) == 16 and ctypes.sizeof(M.Col) == 24
+    assert ctypes.sizeof(M.Out) == 16 and ctypes.sizeof(M.Program) == 16 + 16 * M.MAX_OPS
+    assert M.Col.rows.offset == 16

@@ -201,7 +201,7 @@ def test_bad_programs_are_refused(rf, mctx):
     cols = (M.Col * 1)()
     cols[0].data, cols[0].type = a.ptr, M.TYPES["i32"]
     out = rf.DeviceBuffer(64)
-    outs = (M.Col * 1)()
+    outs = (M.Out * 1)()
     outs[0].data, outs[0].type = out.ptr, M.TYPES["i32"]
     cell = rf.u64_cell()
     rc = rf.lib().rfx_mapfilter(mctx.handle, cols, 1, 10, ctypes.byref(prog), outs, 1, cell.ptr)

@@ -1,0 +1,147 @@
+"""RA plans on the GPU: TPC-H Q1 as the reference's plan builds it
+(q1.man.cpp), a transitive closure that loops until a fixpoint (MOVE,
+COND, UNION, key join with reordering), and set operations, SINGLE,
+PRODUCT, MIN/MAX and a signed filter; results against numpy and Python
+sets computed from the same inputs."""
+import numpy as np
+import pytest
+
+from redfox_amd import raplan
+
+pytestmark = pytest.mark.gpu
+Q = "+$logicQ1:_"
+
+
+def lineitem(n_orders, seed):
+    """Lineitem columns keyed by (LINENUMBER, ORDER), 1-7 lines per order,
+    sorted by the keys as the plan's relations are."""
+    rng = np.random.default_rng(seed)
+    lines = rng.integers(1, 8, n_orders)
+    order = np.repeat(np.arange(1, n_orders + 1, dtype=np.uint32), lines)
+    ln = (np.arange(len(order)) - np.repeat(np.cumsum(lines) - lines, lines) + 1).astype(np.uint8)
+    n = len(order)
+    ship = (rng.integers(8035, 8035 + 2557, n) * 86400).astype(np.int64)
+    rf = rng.integers(0, 3, n).astype(np.uint8)
+    ls = rng.integers(0, 2, n).astype(np.uint8)
+    qty = rng.integers(1, 51, n).astype(np.float64)
+    price = rng.random(n) * 1e4
+    disc = rng.integers(0, 11, n) / 100.0
+    tax = rng.integers(0, 9, n) / 100.0
+    o = np.lexsort((order, ln))
+    return [a[o] for a in (ln, order, ship, rf, ls, qty, price, disc, tax)]
+
+
+@pytest.mark.parametrize("n_orders,delta_days", [(1, 90), (2_000, 90), (150_000, 90),
+                                                 (40_000, 3000)])
+def test_q1_plan(ctx, n_orders, delta_days):
+    ln, od, ship, rf, ls, qty, price, disc, tax = lineitem(n_orders, n_orders)
+    p = raplan.Plan(ctx, raplan.q1_plan())
+    delta = delta_days * 86400
+    p.set(Q + "dateDelta", [np.array([delta])])
+    for name, col in (("L_SHIPDATE", ship), ("L_RETURNFLAG", rf), ("L_LINESTATUS", ls),
+                      ("L_QUANTITY", qty), ("L_EXTENDEDPRICE", price), ("L_DISCOUNT", disc),
+                      ("L_TAX", tax)):
+        p.set(name, [ln, od, col])
+    assert p.run() == 21
+    keep = ship <= 912470400 - delta
+    g = rf.astype(np.int64) * 2 + ls
+    groups = sorted(np.unique(g[keep]).tolist())
+    for out, vals in (("count_order", None), ("sum_qty", qty), ("sum_base_price", price),
+                      ("avg_disc_adt", disc), ("sum_disc_price", price * (1 - disc)),
+                      ("sum_charge", price * (1 - disc) * (1 + tax))):
+        k0, k1, v = p.get(Q + out)
+        assert (k0.astype(np.int64) * 2 + k1).tolist() == groups, out
+        for k, got in zip(groups, v):
+            sel = keep & (g == k)
+            if vals is None:
+                assert got == sel.sum()
+            else:
+                want = vals[sel].sum()
+                assert abs(got - want) <= 1e-12 * abs(want), (out, k)
+
+
+def closure_plan():
+    b = raplan.Builder("closure")
+    for v in ("edge", "path", "next", "new", "flip"):
+        b.var(v, ["uint32"], ["uint32"])
+    b.sequence(0).move("path", "edge").goto(1)
+    s = b.sequence(1)
+    s.mapfilter("flip", "path", 2, b.always(), [b.idx(1, "uint32"), b.idx(0, "uint32")])
+    s.join("new", 1, "flip", "edge", [b.idx(1, "uint32"), b.idx(2, "uint32")])
+    s.setop("next", "union", "path", "new")
+    s.cond("next", "path", 2, 3)
+    b.sequence(3).move("path", "next").goto(1)
+    b.sequence(2).halt()
+    return b.entry_exit(0, 2).bytes()
+
+
+@pytest.mark.parametrize("nodes,edges", [(40, 80), (300, 450)])
+def test_transitive_closure_loop(ctx, nodes, edges):
+    rng = np.random.default_rng(nodes)
+    E = rng.integers(0, nodes, (edges, 2)).astype(np.uint32)
+    E[:, 0] += np.uint32(1 << 31)  # keys above 2^31 keep the unsigned order
+    E[:, 1] += np.uint32(1 << 31)
+    p = raplan.Plan(ctx, closure_plan())
+    p.set("edge", [E[:, 0], E[:, 1]])
+    p.run()
+    got = list(zip(*[c.tolist() for c in p.get("path")]))
+    succ = {}
+    for x, y in E.tolist():
+        succ.setdefault(x, set()).add(y)
+    want = set()
+    for s0 in succ:
+        seen, stack = set(), list(succ[s0])
+        while stack:
+            y = stack.pop()
+            if y not in seen:
+                seen.add(y)
+                stack.extend(succ.get(y, ()))
+        want |= {(s0, y) for y in seen}
+    assert got == sorted(want)
+
+
+def sets_plan():
+    b = raplan.Builder("sets")
+    for v in ("a", "b", "u", "i", "d", "neg"):
+        b.var(v, ["entity:NATION"], ["int32"])
+    b.var("f", ["entity:NATION"], ["float64"])
+    b.var("c", fields=["int64"])
+    b.var("prod", ["entity:NATION"], ["int32", "int64"])
+    b.var("lo", ["entity:NATION"], ["float64"])
+    b.var("hi", ["entity:NATION"], ["float64"])
+    s = b.sequence(0)
+    s.setop("u", "union", "a", "b").setop("i", "intersection", "a", "b")
+    s.setop("d", "difference", "a", "b")
+    s.single("c", [b.const_int(-7)])
+    s.product("prod", "a", "c")
+    s.agg("lo", "f", 2, [0], "min", 1, 1).agg("hi", "f", 2, [0], "max", 1, 1)
+    s.mapfilter("neg", "a", 2, b.and_(b.comp("lt", b.idx(1, "int32"), b.const_int(0)),
+                                      b.comp("ne", b.idx(0, "entity:NATION"), b.const_int(3))),
+                [b.idx(0, "entity:NATION"), b.idx(1, "int32")])
+    s.halt()
+    return b.entry_exit(0, 0).bytes()
+
+
+def test_set_operations_single_product_minmax(ctx):
+    rng = np.random.default_rng(11)
+    A = [(int(k), int(v)) for k, v in zip(rng.integers(0, 25, 3000), rng.integers(-2**31, 2**31, 3000))]
+    A += [(k, v) for k, v in A[:200]]  # duplicates: the set operations see sets
+    B = [(int(k), int(v)) for k, v in zip(rng.integers(0, 25, 2000), rng.integers(-2**31, 2**31, 2000))]
+    B += A[:700]
+    p = raplan.Plan(ctx, sets_plan())
+    p.set("a", [np.array([x for x, _ in A]), np.array([y for _, y in A])])
+    p.set("b", [np.array([x for x, _ in B]), np.array([y for _, y in B])])
+    fk, fv = rng.integers(0, 20, 5000), rng.normal(size=5000)
+    p.set("f", [fk, fv])
+    p.run()
+
+    def rel(name):
+        return list(zip(*[c.tolist() for c in p.get(name)]))
+    assert rel("u") == sorted(set(A) | set(B))
+    assert rel("i") == sorted(set(A) & set(B))
+    assert rel("d") == sorted(set(A) - set(B))
+    assert rel("prod") == [(x, y, -7) for x, y in A]
+    ks = sorted(set(fk.tolist()))
+    assert rel("lo") == [(k, fv[fk == k].min()) for k in ks]
+    assert rel("hi") == [(k, fv[fk == k].max()) for k in ks]
+    assert rel("neg") == [(x, y) for x, y in A if y < 0 and x != 3]
