@@ -294,6 +294,47 @@ def bench_mapfilter(rf, ctx, iters=5):
             "roofline": roofline(28 * n + 16 * kept, launches, ms)}
 
 
+PROF_TAGS = ("join.find_bounds", "join.expand", "join.expand_gather", "join.expand_project",
+             "join.gather", "mapfilter", "q1.partial", "rbk.down", "rbk.up", "reduce.single",
+             "select", "setop", "sort.hist", "sort.onesweep", "unique")
+
+
+def bench_q1_plan(rf, ctx, iters=3, n_orders=1 << 22):
+    """The reference's TPC-H Q1 plan (q1.man.cpp) run by the plan front end
+    over lineitem column relations keyed by (LINENUMBER, ORDER), about
+    2^24 rows (4 per order): a product with the start date, a date
+    filter, five key joins, two arithmetic projections, six AGGs.  Inputs
+    resident in HBM; the timed region is Plan.run() (every node, with the
+    host reading each result size as Harmony does)."""
+    from redfox_amd import datagen, raplan
+
+    cols = datagen.lineitem_keyed(n_orders, SEED + 30)
+    ln, od = cols[0], cols[1]
+    n = len(ln)
+    p = raplan.Plan(ctx, raplan.q1_plan())
+    p.set("+$logicQ1:_dateDelta", [np.array([90 * 86400])])
+    for name, c in zip(("L_SHIPDATE", "L_RETURNFLAG", "L_LINESTATUS", "L_QUANTITY",
+                        "L_EXTENDEDPRICE", "L_DISCOUNT", "L_TAX"), cols[2:]):
+        p.set(name, [ln, od, c])
+    t0 = time.perf_counter()
+    p.run()  # compiles the plan's MAPFILTER programs
+    first_ms = (time.perf_counter() - t0) * 1e3
+    times, kernel_ms = [], []
+    for _ in range(iters):
+        ctx.synchronize()
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        p.run()
+        times.append((time.perf_counter() - t0) * 1e3)
+        kernel_ms.append(sum(ctx.profile_read(t)[1] for t in PROF_TAGS))
+        ctx.profile(False)
+    m = median(times)
+    count = p.get("+$logicQ1:_count_order")[2]
+    return {"n": n, "ms": m, "rows_per_s": n / (m * 1e-3), "first_run_ms": first_ms,
+            "kernel_ms": median(kernel_ms), "commands": p.steps,
+            "kept_rows": int(count.sum()), "groups": int(len(count))}
+
+
 def bench_agg(rf, ctx, iters=5):
     """GROUP BY key: SUM(v), COUNT(*) over 2^28 rows sorted by key (u32
     keys Zipf(1.1) over 2^20 values, f64 values) in one reduce-by-key pass."""
@@ -564,6 +605,18 @@ def cpu_ops_baseline():
     out["agg_2^28"] = line(n, timed(lambda: (oracle.reduce_by_key_f64(g, val, 4, 0),
                                              oracle.reduce_by_key_u32(g, np.ones(n, np.uint32), 4))),
                            "2^22 sorted rows, 2^16 groups, compensated SUM and COUNT")
+    kl = datagen.lineitem_keyed(1 << 20, SEED + 31)
+
+    def q1_plan_np():
+        ship, rf_, ls_, qty, price, disc, tax = kl[2:]
+        keep = ship <= 912470400 - 90 * 86400
+        g = (rf_.astype(np.int64) * 2 + ls_)[keep]
+        dp = price * (1 - disc)
+        for w in (None, qty, price, disc, dp, dp * (1 + tax)):
+            np.bincount(g, weights=None if w is None else w[keep], minlength=6)
+    out["q1_plan_2^24"] = line(len(kl[0]), timed(q1_plan_np),
+                               "about 2^22 keyed lineitem rows: the plan's filter and its six "
+                               "GROUP BY results (numpy, vectorised)")
     t = datagen.splitmix64(SEED + 23, 2 * n)
     out["select_2^27"] = line(n, timed(lambda: oracle.select(t, 16, 64, 32, 2, 1 << 31)),
                               "2^22 16-byte tuples, half kept")
@@ -609,6 +662,7 @@ def main():
         ("agg_2^28", "group-by sum+count 2^28", lambda: bench_agg(rf, ctx)),
         ("select_2^27", "select 2^27 16-byte tuples", lambda: bench_select(rf, ctx)),
         ("mapfilter_2^27", "mapfilter 2^27 lineitem rows", lambda: bench_mapfilter(rf, ctx)),
+        ("q1_plan_2^24", "Q1 plan over 2^24 keyed lineitem rows", lambda: bench_q1_plan(rf, ctx)),
         ("setop_2^27", "set operations 2^27 + 2^27 u64", lambda: bench_setop(rf, ctx)),
         ("batched_512", "batched 512 join-aggregate instances", lambda: bench_batched(rf, ctx)),
         ("served_mix", "served mixed requests", lambda: bench_served(rf, ctx)),

@@ -102,3 +102,25 @@ def lineitem(seed: int, n: int):
     disc = (((b >> np.uint64(16)) & np.uint64(0xFFFF)) % np.uint64(11)).astype(np.float64) / 100.0
     tax = ((b & np.uint64(0xFFFF)) % np.uint64(9)).astype(np.float64) / 100.0
     return (shipdate, flag.astype(np.uint8), status.astype(np.uint8), qty, price, disc, tax)
+
+
+def lineitem_keyed(n_orders: int, seed: int):
+    """Lineitem as column relations keyed by (LINENUMBER, ORDER), the form
+    the reference's Q1 plan reads (q1.man.cpp): 1-7 lines per order,
+    sorted by the keys.  Returns (linenumber u8, order u32, shipdate i64
+    seconds over 1992-1998, returnflag u8 0-2, linestatus u8 0-1, quantity,
+    extendedprice, discount, tax as f64)."""
+    rng = np.random.default_rng(seed)
+    lines = rng.integers(1, 8, n_orders)
+    order = np.repeat(np.arange(1, n_orders + 1, dtype=np.uint32), lines)
+    ln = (np.arange(len(order)) - np.repeat(np.cumsum(lines) - lines, lines) + 1).astype(np.uint8)
+    n = len(order)
+    ship = (rng.integers(8035, 8035 + 2557, n) * 86400).astype(np.int64)
+    rf = rng.integers(0, 3, n).astype(np.uint8)
+    ls = rng.integers(0, 2, n).astype(np.uint8)
+    qty = rng.integers(1, 51, n).astype(np.float64)
+    price = rng.random(n) * 1e4
+    disc = rng.integers(0, 11, n) / 100.0
+    tax = rng.integers(0, 9, n) / 100.0
+    o = np.lexsort((order, ln))
+    return [a[o] for a in (ln, order, ship, rf, ls, qty, price, disc, tax)]

@@ -286,14 +286,48 @@ def _num_type(nt) -> str:
 
 
 class Relation:
-    """Columns on the device (declaration order) and a row count."""
+    """Columns on the device (declaration order), a row count, and how many
+    leading columns the rows are sorted by (lexicographically, by value),
+    as the reference tracks isSorted (RelationalAlgebraCompiler.cpp:
+    7376-7400)."""
 
-    def __init__(self, cols: List[Tuple[object, str]], n: int):
-        self.cols, self.n = cols, n
+    def __init__(self, cols: List[Tuple[object, str]], n: int, sorted_by: int = 0):
+        self.cols, self.n, self.sorted_by = cols, n, sorted_by
 
     @property
     def types(self) -> List[str]:
         return [t for _, t in self.cols]
+
+
+def sorted_prefix(columns: Sequence[np.ndarray]) -> int:
+    """How many leading integer columns the rows are sorted by."""
+    n = len(columns[0]) if columns else 0
+    if n < 2:
+        return sum(1 for c in columns if np.asarray(c).dtype.kind in "iu") if n else 0
+    lt = np.zeros(n - 1, dtype=bool)
+    eq = np.ones(n - 1, dtype=bool)
+    s = 0
+    for c in columns:
+        c = np.asarray(c)
+        if c.dtype.kind not in "iu":
+            break
+        a, b = c[:-1], c[1:]
+        if not np.all(lt | (eq & (a <= b))):
+            break
+        s += 1
+        lt |= eq & (a < b)
+        eq &= a == b
+    return s
+
+
+def _identity_prefix(exps, limit: int) -> int:
+    """Leading reordering expressions that are Index(i) at position i < limit."""
+    k = 0
+    for i, e in enumerate(exps):
+        if i >= limit or e.tag != 1 or e.index.offset != i:
+            break
+        k += 1
+    return k
 
 
 # ---------------------------------------------------------------------------
@@ -469,10 +503,13 @@ class Plan:
         self.ctx, self.graph, self.max_steps = ctx, parse(data), max_steps
         self.decl: Dict[str, List[str]] = {}
         self.rel: Dict[str, Relation] = {}
+        # Device buffers by power-of-two capacity, reused across nodes and
+        # runs (freeing device memory synchronises the device).
+        self._free: Dict[int, List[object]] = {}
         for v in self.graph.variables:
             types = [column_type(t) for t in list(v.keys) + list(v.fields)]
             self.decl[v.varName] = types
-            self.rel[v.varName] = Relation([(rfx.DeviceBuffer(8), t) for t in types], 0)
+            self.rel[v.varName] = self._empty(types)
         self.steps = 0
 
     # -- relations --------------------------------------------------------
@@ -481,13 +518,17 @@ class Plan:
         if len(columns) != len(types):
             raise ValueError(f"{name} has {len(types)} columns")
         n = len(columns[0]) if columns else 0
-        cols = []
+        cols, host = [], []
         for a, t in zip(columns, types):
             a = np.ascontiguousarray(a, dtype=_NP[t])
             if len(a) != n:
                 raise ValueError("columns of one relation have one length")
-            cols.append((rfx.DeviceBuffer.from_numpy(a) if n else rfx.DeviceBuffer(8), t))
-        self.rel[name] = Relation(cols, n)
+            b = self._buf(a.nbytes)
+            if n:
+                b.upload(a)
+            cols.append((b, t))
+            host.append(a)
+        self._replace(name, Relation(cols, n, sorted_prefix(host)))
 
     def get(self, name: str) -> List[np.ndarray]:
         r = self.rel[name]
@@ -497,9 +538,38 @@ class Plan:
     def rows(self, name: str) -> int:
         return self.rel[name].n
 
-    # -- helpers ------------------------------------------------------------
+    # -- device buffers -----------------------------------------------------
+    def _buf(self, nbytes: int):
+        cap = 256
+        while cap < nbytes:
+            cap *= 2
+        free = self._free.get(cap)
+        return free.pop() if free else rfx.DeviceBuffer(cap)
+
     def _alloc(self, t: str, n: int):
-        return rfx.DeviceBuffer(max(8, n * np.dtype(_NP[t]).itemsize))
+        return self._buf(n * np.dtype(_NP[t]).itemsize)
+
+    def _cell(self, value: int = 0):
+        c = self._buf(8)
+        c.upload(np.array([value], dtype=np.uint64))
+        return c
+
+    def _release(self, *bufs) -> None:
+        """Return buffers to the pool (work already queued on the stream
+        that reads them runs before any later reuse)."""
+        for b in bufs:
+            if b is not None:
+                self._free.setdefault(b.nbytes, []).append(b)
+
+    def _empty(self, types: Sequence[str]) -> Relation:
+        return Relation([(self._buf(8), t) for t in types], 0, len(types))
+
+    def _replace(self, name: str, rel: Relation) -> None:
+        old = self.rel.get(name)
+        self.rel[name] = rel
+        if old is not None and old is not rel:
+            keep = {id(b) for b, _ in rel.cols}
+            self._release(*[b for b, _ in old.cols if id(b) not in keep])
 
     def _mapfilter(self, columns, n, build, out_types) -> Relation:
         """One MAPFILTER pass: build(x) sets the filter and outputs over
@@ -511,9 +581,10 @@ class Plan:
         x = M.MapFilter(columns)
         build(x)
         outs = [self._alloc(t, n) for t in out_types]
-        cell = rfx.u64_cell()
+        cell = self._buf(8)
         x.run(self.ctx, n, outs, cell)
         kept = rfx.read_u64(cell)
+        self._release(cell)
         return Relation(list(zip(outs, out_types)), kept)
 
     def _mapfilter_wide(self, columns, n, build, out_types) -> Relation:
@@ -528,19 +599,21 @@ class Plan:
             kept = r.n if kept is None else kept
         return Relation(rel_cols, kept or 0)
 
-    def _pack(self, rel: Relation, idx: Sequence[int], rows_out=False):
-        """(sorted keys, row permutation, key type, key bits, layout) of the
-        packed key over columns idx."""
+    def _pack(self, rel: Relation, idx: Sequence[int]):
+        """(sorted keys, row permutation or None, key type, key bits,
+        layout) of the packed key over columns idx; no sort (and no
+        permutation) when the rows are already sorted by those columns."""
         kt, bits, fields = key_layout([rel.types[i] for i in idx])
         n = rel.n
-        keys = self._alloc(kt, n)
+        src = [(rel.cols[i][0], rel.cols[i][1]) for i in idx]
+        keys = self._mapfilter(src, n, lambda x: x.emit(_pack_expr(x.cols, fields), kt),
+                               [kt]).cols[0][0]
+        presorted = list(idx) == list(range(len(idx))) and rel.sorted_by >= len(idx)
+        if presorted:
+            return keys, None, kt, bits, fields
         perm = self._alloc("u32", n)
-        if n:
-            src = [(rel.cols[i][0], rel.cols[i][1]) for i in idx]
-            r = self._mapfilter(src, n, lambda x: x.emit(_pack_expr(x.cols, fields), kt), [kt])
-            keys = r.cols[0][0]
-            self.ctx.iota_u32(perm, n)
-            self.ctx.sort_pairs(keys, perm, n, 4 if kt == "u32" else 8, bits, 4)
+        self.ctx.iota_u32(perm, n)
+        self.ctx.sort_pairs(keys, perm, n, 4 if kt == "u32" else 8, bits, 4)
         return keys, perm, kt, bits, fields
 
     def _result(self, dest: str, rel: Relation) -> None:
@@ -550,11 +623,13 @@ class Plan:
             if len(types) != len(rel.types):
                 raise ValueError(f"{dest}: {len(rel.types)} columns computed, "
                                  f"{len(types)} declared")
-            cols = list(rel.cols)
-            rel = self._mapfilter(cols, rel.n, lambda x: [
+            conv = self._mapfilter(list(rel.cols), rel.n, lambda x: [
                 x.emit(_as(c, t), t) for c, t in zip(x.cols, types)], types) if rel.n else \
-                Relation([(rfx.DeviceBuffer(8), t) for t in types], 0)
-        self.rel[dest] = rel
+                self._empty(types)
+            conv.sorted_by = rel.sorted_by
+            self._release(*[b for b, _ in rel.cols])
+            rel = conv
+        self._replace(dest, rel)
         self.decl.setdefault(dest, rel.types)
 
     # -- operators ----------------------------------------------------------
@@ -563,6 +638,7 @@ class Plan:
         if node.srcWidth and node.srcWidth != len(src.cols):
             raise ValueError(f"MAPFILTER over {node.srcA}: width {node.srcWidth}, "
                              f"relation has {len(src.cols)} columns")
+
         def build(x):
             col = lambda i: (x.cols[i], src.types[i])  # noqa: E731
             f = lower_compare(node.predicate, col)
@@ -576,8 +652,10 @@ class Plan:
         probe = M.MapFilter([(b, t) for b, t in src.cols])
         types = [lower_exp(e, lambda i: (probe.cols[i], src.types[i]))[1] for e in node.reordering]
         if src.n == 0:
-            return Relation([(rfx.DeviceBuffer(8), t) for t in types], 0)
-        return self._mapfilter(list(src.cols), src.n, build, types)
+            return self._empty(types)
+        out = self._mapfilter(list(src.cols), src.n, build, types)
+        out.sorted_by = _identity_prefix(node.reordering, src.sorted_by)  # a filter keeps order
+        return out
 
     def join(self, key_count: int, a_name: str, b_name: str, args) -> Relation:
         A, B = self.rel[a_name], self.rel[b_name]
@@ -587,39 +665,44 @@ class Plan:
         if [A.types[i] for i in range(k)] != [B.types[i] for i in range(k)]:
             raise ValueError("joined key columns differ in type")
         comb = A.types + B.types[k:]
-        arg_types = None
         if args:
-            probe = M.MapFilter([(rfx.DeviceBuffer(8), t) for t in comb])
-            arg_types = [lower_exp(e, lambda i: (probe.cols[i], comb[i]))[1] for e in args]
-        out_types = arg_types if args else comb
+            probe = M.MapFilter([(None, t) for t in comb])
+            out_types = [lower_exp(e, lambda i: (probe.cols[i], comb[i]))[1] for e in args]
+        else:
+            out_types = comb
         if A.n == 0 or B.n == 0:
-            return Relation([(rfx.DeviceBuffer(8), t) for t in out_types], 0)
+            return self._empty(out_types)
         ka, pa, kt, _, _ = self._pack(A, range(k))
         kb, pb, _, _, _ = self._pack(B, range(k))
         kbytes = 4 if kt == "u32" else 8
-        lb, lc = self._alloc("u32", A.n), self._alloc("u32", A.n)
-        tot = rfx.u64_cell()
+        lb, lc, tot = self._alloc("i32", A.n), self._alloc("i32", A.n), self._buf(8)
         self.ctx.join_find_bounds(lb, lc, tot, ka, A.n, kb, B.n, kbytes)
         total = rfx.read_u64(tot)
-        if total == 0:
-            return Relation([(rfx.DeviceBuffer(8), t) for t in out_types], 0)
         if total >= 1 << 31:
+            self._release(ka, kb, pa, pb, lb, lc, tot)
             raise ValueError(f"join of {total} rows exceeds the int32 row index range")
+        if total == 0:
+            self._release(ka, kb, pa, pb, lb, lc, tot)
+            return self._empty(out_types)
         li, ri = self._alloc("i32", total), self._alloc("i32", total)
         self.ctx.join_main(li, ri, tot, total, lb, lc, A.n)
-        # One pass: the combined tuple through the row pairs.
-        cols = [(li, "i32"), (ri, "i32"), (pa, "u32", A.n), (pb, "u32", B.n)]
+        # One pass: the combined tuple through the row pairs (through each
+        # side's sort permutation where it was sorted here).
+        cols = [(li, "i32"), (ri, "i32")]
+        if pa is not None:
+            cols.append((pa, "u32", A.n))
+        if pb is not None:
+            cols.append((pb, "u32", B.n))
+        first = len(cols)
         cols += [(b, t, A.n) for b, t in A.cols] + [(b, t, B.n) for b, t in B.cols[k:]]
         na = len(A.cols)
 
         def build(x):
-            ra = x.gather(2, x.cols[0])
-            rb = x.gather(3, x.cols[1])
+            ra = x.cols[0] if pa is None else x.gather(2, x.cols[0])
+            rb = x.cols[1] if pb is None else x.gather(2 + (pa is not None), x.cols[1])
 
             def col(i):
-                if i < na:
-                    return x.gather(4 + i, ra), comb[i]
-                return x.gather(4 + i, rb), comb[i]
+                return x.gather(first + i, ra if i < na else rb), comb[i]
             if args:
                 for e in args:
                     v, t = lower_exp(e, col)
@@ -628,7 +711,13 @@ class Plan:
                 for i in range(len(comb)):
                     v, t = col(i)
                     x.emit(v, t)
-        return self._mapfilter(cols, total, build, out_types)
+        out = self._mapfilter(cols, total, build, out_types)
+        self._release(ka, kb, pa, pb, lb, lc, tot, li, ri)
+        # Pairs come in key order, left rows in their order within a key:
+        # sorted by A's sorted columns where A needed no sort, else by the key.
+        s_out = A.sorted_by if pa is None else k
+        out.sorted_by = _identity_prefix(args, s_out) if args else s_out
+        return out
 
     def agg(self, node) -> Relation:
         src = self.rel[node.srcA]
@@ -643,11 +732,12 @@ class Plan:
             raise ValueError("TOTAL/MIN/MAX run on floating-point columns (FP64 sums)")
         dom_types = [src.types[i] for i in dom]
         if src.n == 0:
-            return Relation([(rfx.DeviceBuffer(8), t) for t in dom_types + [out_t]], 0)
+            return self._empty(dom_types + [out_t])
         kt, bits, fields = key_layout(dom_types)
         kbytes = 4 if kt == "u32" else 8
         n = src.n
-        # keys (and values as f64) in one pass, then sort them together.
+        # keys (and values as f64) in one pass, then sort them together
+        # unless the rows are already in domain order.
         srcc = [(src.cols[i][0], src.cols[i][1]) for i in dom]
         if vcol is not None:
             srcc.append(src.cols[vcol])
@@ -659,26 +749,28 @@ class Plan:
         kv = self._mapfilter(srcc, n, build, [kt] + (["f64"] if vcol is not None else []))
         keys = kv.cols[0][0]
         vals = kv.cols[1][0] if vcol is not None else None
-        self.ctx.sort_pairs(keys, vals, n, kbytes, bits, 8 if vals is not None else 0)
-        ncell, nout = rfx.u64_cell(n), rfx.u64_cell()
+        if not (dom == list(range(len(dom))) and src.sorted_by >= len(dom)):
+            self.ctx.sort_pairs(keys, vals, n, kbytes, bits, 8 if vals is not None else 0)
+        ncell, nout = self._cell(n), self._buf(8)
         okeys = self._alloc(kt, n)
         if which == "count":
-            ovals = self._alloc("u32", n)
+            ovals, vt = self._alloc("u32", n), "u32"
             self.ctx.count_by_key(okeys, ovals, nout, keys, ncell, n, kbytes)
-            vt = "u32"
         else:
-            ovals = self._alloc("f64", n)
+            ovals, vt = self._alloc("f64", n), "f64"
             op = {"total": 0, "min": 1, "max": 2}[which]
             self.ctx.reduce_by_key_f64(okeys, ovals, nout, keys, vals, ncell, n, kbytes, op)
-            vt = "f64"
         groups = rfx.read_u64(nout)
 
         def unpack(x):
             for f, t in zip(fields, dom_types):
                 x.emit(_unpack_expr(x.cols[0], f, bits), t)
             x.emit(_as(x.cols[1], out_t), out_t)
-        return self._mapfilter([(okeys, _signed(kt)), (ovals, vt)], groups, unpack,
-                               dom_types + [out_t])
+        out = self._mapfilter([(okeys, _signed(kt)), (ovals, vt)], groups, unpack,
+                              dom_types + [out_t])
+        self._release(keys, vals, ncell, nout, okeys, ovals)
+        out.sorted_by = len(dom)
+        return out
 
     def set_operation(self, op: int, a_name: str, b_name: str) -> Relation:
         """op: 0 union, 1 difference, 2 intersection, on whole tuples."""
@@ -687,43 +779,53 @@ class Plan:
             raise ValueError("set operations need inputs of one type")
         kt, bits, fields = key_layout(A.types)
         kbytes = 4 if kt == "u32" else 8
-        sets = []
+        sets, tmp = [], []
         for R in (A, B):
             if R.n == 0:
                 sets.append((self._alloc(kt, 1), 0))
+                tmp.append(sets[-1][0])
                 continue
-            keys, _, _, _, _ = self._pack(R, range(len(R.cols)))
-            uniq, cnt = self._alloc(kt, R.n), rfx.u64_cell()
+            keys, perm, _, _, _ = self._pack(R, range(len(R.cols)))
+            uniq, cnt = self._alloc(kt, R.n), self._buf(8)
             self.ctx.unique(uniq, cnt, keys, R.n, kbytes)
             sets.append((uniq, rfx.read_u64(cnt)))
+            tmp += [keys, perm, uniq, cnt]
         (a, na), (b, nb) = sets
-        out, cnt = self._alloc(kt, na + nb + 1), rfx.u64_cell()
+        out, cnt = self._alloc(kt, na + nb + 1), self._buf(8)
         self.ctx.set_op(op, out, cnt, a, na, b, nb, kbytes)
         m = rfx.read_u64(cnt)
         if m == 0:
-            return Relation([(rfx.DeviceBuffer(8), t) for t in A.types], 0)
-        return self._mapfilter([(out, _signed(kt))], m, lambda x: [
-            x.emit(_unpack_expr(x.cols[0], f, bits), t) for f, t in zip(fields, A.types)], A.types)
+            res = self._empty(A.types)
+        else:
+            res = self._mapfilter([(out, _signed(kt))], m, lambda x: [
+                x.emit(_unpack_expr(x.cols[0], f, bits), t) for f, t in zip(fields, A.types)],
+                A.types)
+        self._release(out, cnt, *tmp)
+        res.sorted_by = len(A.types)
+        return res
 
     def single(self, dest: str, elements) -> Relation:
         vals = [_constant(c) for c in elements]
         types = self.decl.get(dest) or [t for _, t in vals]
         if len(types) != len(vals):
             raise ValueError(f"SINGLE of {len(vals)} constants for {len(types)} columns")
-        cols = [(rfx.DeviceBuffer.from_numpy(np.array([v], dtype=_NP[t])), t)
-                for (v, _), t in zip(vals, types)]
-        return Relation(cols, 1)
+        cols = []
+        for (v, _), t in zip(vals, types):
+            b = self._buf(8)
+            b.upload(np.array([v], dtype=_NP[t]))
+            cols.append((b, t))
+        return Relation(cols, 1, len(types))
 
     def copy(self, name: str) -> Relation:
         r = self.rel[name]
         cols = []
         for b, t in r.cols:
             nb = r.n * np.dtype(_NP[t]).itemsize
-            c = rfx.DeviceBuffer(max(nb, 8))
+            c = self._buf(nb)
             if nb:
                 self.ctx.copy(c, b, nb)
             cols.append((c, t))
-        return Relation(cols, r.n)
+        return Relation(cols, r.n, r.sorted_by)
 
     def equal(self, a: str, b: str) -> bool:
         A, B = self.rel[a], self.rel[b]
@@ -774,7 +876,7 @@ class Plan:
                     self.assign(cmd.assign)
                     continue
                 if t == 5:
-                    self.rel[cmd.move.dest] = self.copy(cmd.move.src1)
+                    self._replace(cmd.move.dest, self.copy(cmd.move.src1))
                     continue
                 if t == 4:
                     target = cmd.jump.target

@@ -6,35 +6,16 @@ sets computed from the same inputs."""
 import numpy as np
 import pytest
 
-from redfox_amd import raplan
+from redfox_amd import datagen, raplan
 
 pytestmark = pytest.mark.gpu
 Q = "+$logicQ1:_"
 
 
-def lineitem(n_orders, seed):
-    """Lineitem columns keyed by (LINENUMBER, ORDER), 1-7 lines per order,
-    sorted by the keys as the plan's relations are."""
-    rng = np.random.default_rng(seed)
-    lines = rng.integers(1, 8, n_orders)
-    order = np.repeat(np.arange(1, n_orders + 1, dtype=np.uint32), lines)
-    ln = (np.arange(len(order)) - np.repeat(np.cumsum(lines) - lines, lines) + 1).astype(np.uint8)
-    n = len(order)
-    ship = (rng.integers(8035, 8035 + 2557, n) * 86400).astype(np.int64)
-    rf = rng.integers(0, 3, n).astype(np.uint8)
-    ls = rng.integers(0, 2, n).astype(np.uint8)
-    qty = rng.integers(1, 51, n).astype(np.float64)
-    price = rng.random(n) * 1e4
-    disc = rng.integers(0, 11, n) / 100.0
-    tax = rng.integers(0, 9, n) / 100.0
-    o = np.lexsort((order, ln))
-    return [a[o] for a in (ln, order, ship, rf, ls, qty, price, disc, tax)]
-
-
 @pytest.mark.parametrize("n_orders,delta_days", [(1, 90), (2_000, 90), (150_000, 90),
                                                  (40_000, 3000)])
 def test_q1_plan(ctx, n_orders, delta_days):
-    ln, od, ship, rf, ls, qty, price, disc, tax = lineitem(n_orders, n_orders)
+    ln, od, ship, rf, ls, qty, price, disc, tax = datagen.lineitem_keyed(n_orders, n_orders)
     p = raplan.Plan(ctx, raplan.q1_plan())
     delta = delta_days * 86400
     p.set(Q + "dateDelta", [np.array([delta])])
