@@ -626,6 +626,34 @@ int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, uint64_t n_r
                   uint64_t* n_kept);
 
 /* ---------------------------------------------------------------------- */
+/* Loading delimited text                                                 */
+/* ---------------------------------------------------------------------- */
+/* Rows of delimited text (TPC-H .tbl: fields ended by '|'; the reference's
+ * TC.tbl: tab-separated) already in device memory, to typed device
+ * columns (TC_data_process.cu:20-100: getline, strtok, atoi).  Rows end at
+ * '\n' (a '\r' before it is dropped); a final row needs no newline.  Field
+ * f of every row goes to columns[f] as types[f]:
+ *   RFX_TBL_SKIP  not stored            RFX_TBL_I32/I64/U32/U8  atoi-style
+ *   RFX_TBL_F64   decimal digits[.digits] (mantissa / 10^k, correctly
+ *                 rounded below 2^53)   RFX_TBL_DATE  YYYY-MM-DD as i64
+ *                 seconds (Date.h)      RFX_TBL_CHAR  the field's first byte
+ * *rows (device u64) gets the row count; rows past `capacity` are counted,
+ * not stored.  *short_rows (device u64) counts rows with fewer fields than
+ * n_fields (their missing fields read 0).  Synchronises the stream. */
+#define RFX_TBL_MAX_COLS 32
+#define RFX_TBL_SKIP 0
+#define RFX_TBL_I32 1
+#define RFX_TBL_I64 2
+#define RFX_TBL_U32 3
+#define RFX_TBL_U8 4
+#define RFX_TBL_F64 5
+#define RFX_TBL_DATE 6
+#define RFX_TBL_CHAR 7
+int rfx_tbl_parse(rfx_ctx* ctx, const void* text, uint64_t bytes, char delim,
+                  const int* types, int n_fields, void* const* columns,
+                  uint64_t capacity, uint64_t* rows, uint64_t* short_rows);
+
+/* ---------------------------------------------------------------------- */
 /* Harmony IR programs                                                    */
 /* ---------------------------------------------------------------------- */
 /* A loaded HIR file: a u64 byte count, a KernelControlFlowGraph message and

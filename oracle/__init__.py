@@ -405,3 +405,89 @@ def mapfilter(filter_tree, outputs, cols, n):
             v = bits.astype(np.dtype(f"u{np.dtype(np_t[t]).itemsize}")).view(np_t[t])
         outs.append(np.ascontiguousarray(v))
     return keep, outs
+
+
+def _atoi(field: bytes) -> int:
+    """C atoi/atol on one field: leading spaces, a sign, digits up to the
+    first other byte (TC_data_process.cu:60-64 calls atoi)."""
+    i, n = 0, len(field)
+    while i < n and field[i:i + 1] == b" ":
+        i += 1
+    neg = False
+    if i < n and field[i:i + 1] in (b"-", b"+"):
+        neg = field[i:i + 1] == b"-"
+        i += 1
+    v = 0
+    while i < n and 48 <= field[i] <= 57:
+        v = v * 10 + field[i] - 48
+        i += 1
+    return -v if neg else v
+
+
+def _decimal(field: bytes) -> float:
+    """digits[.digits] with an optional sign: Python's correctly rounded
+    conversion of the same digits."""
+    i, n = 0, len(field)
+    while i < n and field[i:i + 1] == b" ":
+        i += 1
+    j = i
+    if j < n and field[j:j + 1] in (b"-", b"+"):
+        j += 1
+    seen_point = False
+    while j < n and (48 <= field[j] <= 57 or (field[j:j + 1] == b"." and not seen_point)):
+        seen_point |= field[j:j + 1] == b"."
+        j += 1
+    text = field[i:j].decode()
+    if text in ("", "-", "+", ".", "-.", "+."):
+        return -0.0 if text.startswith("-") else 0.0
+    return float(text + ("0" if text.endswith(".") else ""))
+
+
+def parse_tbl(text: bytes, spec, delim: bytes = b"|"):
+    """Rows of delimited text to numpy columns (one per non-skipped spec
+    entry): rows end at a newline (a final row needs none, a carriage
+    return before the newline is dropped); missing fields read 0.  Types:
+    i32/i64/u32/u8 (atoi, wrapping to the width), f64 (decimal), date
+    (YYYY-MM-DD -> seconds, Date.h), char (first byte).  Returns (columns,
+    rows with missing fields)."""
+    lines = text.split(b"\n")
+    if lines and lines[-1] == b"":
+        lines.pop()
+    np_t = {"i32": np.int32, "i64": np.int64, "u32": np.uint32, "u8": np.uint8,
+            "f64": np.float64, "date": np.int64, "char": np.uint8}
+    out = {f: [] for f, s in enumerate(spec) if s != "skip"}
+    short = 0
+    for line in lines:
+        if line.endswith(b"\r"):
+            line = line[:-1]
+        fields = line.split(delim)
+        if len(fields) < len(spec):
+            short += 1
+        for f, s in enumerate(spec):
+            if s == "skip":
+                continue
+            fld = fields[f] if f < len(fields) else None
+            if fld is None:
+                v = 0
+            elif s == "f64":
+                v = _decimal(fld)
+            elif s == "char":
+                v = fld[0] if fld else 0
+            elif s == "date":
+                parts = (fld.split(b"-") + [b"", b"", b""])[:3]
+                v = date2int(*[_atoi(p) for p in parts])
+            else:
+                v = _atoi(fld)
+            out[f].append(v)
+    cols = []
+    for f, s in enumerate(spec):
+        if s == "skip":
+            continue
+        if s == "f64":
+            cols.append(np.array(out[f], dtype=np.float64))
+        else:
+            bits = np.array([v & 0xFFFFFFFFFFFFFFFF for v in out[f]], dtype=np.uint64)
+            cols.append(bits.astype(np.dtype(np_t[s]).newbyteorder("=")) if s in ("u8", "char")
+                        else bits.view(np.int64).astype(np_t[s]) if s != "u32"
+                        else bits.astype(np.uint32))
+    return cols, short

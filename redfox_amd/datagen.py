@@ -124,3 +124,50 @@ def lineitem_keyed(n_orders: int, seed: int):
     tax = rng.integers(0, 9, n) / 100.0
     o = np.lexsort((order, ln))
     return [a[o] for a in (ln, order, ship, rf, ls, qty, price, disc, tax)]
+
+
+def _ascii_digits(v: np.ndarray, width: int) -> np.ndarray:
+    """Zero-padded decimal digits of non-negative integers, (n, width) ASCII."""
+    out = np.empty((len(v), width), dtype=np.uint8)
+    x = v.astype(np.uint64).copy()
+    for i in range(width - 1, -1, -1):
+        out[:, i] = 48 + (x % np.uint64(10)).astype(np.uint8)
+        x //= np.uint64(10)
+    return out
+
+
+def lineitem_tbl(n_orders: int, seed: int):
+    """dbgen-style lineitem.tbl text ('|' after every field, rows in
+    (orderkey, linenumber) order, fixed-width numbers) and the values it
+    encodes: (text, order u32, linenumber u8, quantity, extendedprice,
+    discount, tax f64, returnflag, linestatus u8 chars, shipdate i64
+    seconds).  Decimal values are the correctly rounded value of their text
+    (cents / 100)."""
+    rng = np.random.default_rng(seed)
+    lines = rng.integers(1, 8, n_orders)
+    order = np.repeat(np.arange(1, n_orders + 1, dtype=np.uint32), lines)
+    ln = (np.arange(len(order)) - np.repeat(np.cumsum(lines) - lines, lines) + 1).astype(np.uint8)
+    n = len(order)
+    qty = rng.integers(1, 51, n)
+    cents = rng.integers(90_000, 10_500_000, n)
+    disc = rng.integers(0, 11, n)
+    tax = rng.integers(0, 9, n)
+    rf = np.frombuffer(b"ANR", dtype=np.uint8)[rng.integers(0, 3, n)]
+    ls = np.frombuffer(b"FO", dtype=np.uint8)[rng.integers(0, 2, n)]
+    day = rng.integers(8035, 8035 + 2557, n)
+    dates = np.datetime_as_string(np.datetime64("1970-01-01") + day.astype("timedelta64[D]"))
+    date_bytes = np.char.encode(dates).view(np.uint8).reshape(n, 10)
+    bar = np.full((n, 1), ord("|"), dtype=np.uint8)
+    dot = np.full((n, 1), ord("."), dtype=np.uint8)
+    zero = np.full((n, 1), ord("0"), dtype=np.uint8)
+    tail = np.frombuffer(b"DELIVER IN PERSON|TRUCK|carefully final deposits|", np.uint8)
+    parts = [_ascii_digits(order, 8), bar, _ascii_digits(rng.integers(1, 200_000, n), 6), bar,
+             _ascii_digits(rng.integers(1, 10_000, n), 5), bar, _ascii_digits(ln, 1), bar,
+             _ascii_digits(qty, 2), dot, zero, zero, bar,
+             _ascii_digits(cents // 100, 6), dot, _ascii_digits(cents % 100, 2), bar,
+             zero, dot, _ascii_digits(disc, 2), bar, zero, dot, _ascii_digits(tax, 2), bar,
+             rf[:, None], bar, ls[:, None], bar, date_bytes, bar, date_bytes, bar, date_bytes, bar,
+             np.broadcast_to(tail, (n, len(tail))), np.full((n, 1), 10, dtype=np.uint8)]
+    text = np.concatenate(parts, axis=1).tobytes()
+    return (text, order, ln, qty.astype(np.float64), cents / 100.0, disc / 100.0, tax / 100.0,
+            rf, ls, (day * 86400).astype(np.int64))

@@ -530,6 +530,22 @@ class Plan:
             host.append(a)
         self._replace(name, Relation(cols, n, sorted_prefix(host)))
 
+    def set_columns(self, name: str, columns: Sequence[Tuple[object, str]], n: int,
+                    sorted_by: int = 0) -> None:
+        """Set a relation from device columns (copied into the plan's own
+        buffers); sorted_by: leading columns the rows are known sorted by."""
+        types = self.decl[name]
+        if [t for _, t in columns] != types:
+            raise ValueError(f"{name} has columns {types}")
+        cols = []
+        for b, t in columns:
+            nb = n * np.dtype(_NP[t]).itemsize
+            c = self._buf(nb)
+            if nb:
+                self.ctx.copy(c, b, nb)
+            cols.append((c, t))
+        self._replace(name, Relation(cols, n, sorted_by))
+
     def get(self, name: str) -> List[np.ndarray]:
         r = self.rel[name]
         self.ctx.synchronize()
