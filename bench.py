@@ -335,6 +335,39 @@ def bench_q1_plan(rf, ctx, iters=3, n_orders=1 << 22):
             "kept_rows": int(count.sum()), "groups": int(len(count))}
 
 
+def bench_tbl(rf, ctx, iters=3, n_orders=1 << 20):
+    """lineitem.tbl text (about 4M dbgen-style rows, 147 bytes each)
+    resident in HBM to the Q1 plan's typed columns: a row-count pass, then
+    the row-ends pass and the field pass into columns of that size."""
+    from redfox_amd import datagen, loaders
+
+    text = datagen.lineitem_tbl(n_orders, SEED + 40)[0]
+    raw = np.frombuffer(text, dtype=np.uint8)
+    buf = rf.DeviceBuffer.from_numpy(raw)
+    spec = loaders.LINEITEM_SPEC
+    loaders.parse_tbl(ctx, buf, spec, "|", len(raw))
+    ctx.synchronize()
+    ctx.profile(True)
+    times = []
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        cols, n, _ = loaders.parse_tbl(ctx, buf, spec, "|", len(raw))
+        times.append((time.perf_counter() - t0) * 1e3)
+        for c in cols:
+            if c is not None:
+                c.close()
+    rows_l, rows_ms = ctx.profile_read("tbl.rows")
+    fld_l, fld_ms = ctx.profile_read("tbl.fields")
+    ctx.profile(False)
+    buf.close()
+    m = median(times)
+    out_bytes = sum(loaders.WIDTH[s] for s in spec)
+    return {"bytes": len(raw), "rows": n, "ms": m, "GBps_text": len(raw) / (m * 1e-3) / 1e9,
+            "rows_per_s": n / (m * 1e-3),
+            "rows_pass": roofline(len(raw) + 8 * n, rows_l, rows_ms),
+            "fields_pass": roofline(len(raw) + (out_bytes + 16) * n, fld_l, fld_ms)}
+
+
 def bench_agg(rf, ctx, iters=5):
     """GROUP BY key: SUM(v), COUNT(*) over 2^28 rows sorted by key (u32
     keys Zipf(1.1) over 2^20 values, f64 values) in one reduce-by-key pass."""
@@ -605,6 +638,12 @@ def cpu_ops_baseline():
     out["agg_2^28"] = line(n, timed(lambda: (oracle.reduce_by_key_f64(g, val, 4, 0),
                                              oracle.reduce_by_key_u32(g, np.ones(n, np.uint32), 4))),
                            "2^22 sorted rows, 2^16 groups, compensated SUM and COUNT")
+    from redfox_amd import loaders
+
+    tt = datagen.lineitem_tbl(1 << 14, SEED + 41)[0]
+    out["tbl_lineitem_4M"] = line(tt.count(b"\n"), timed(lambda: oracle.parse_tbl(tt, loaders.LINEITEM_SPEC)),
+                                  "about 2^16 dbgen-style lineitem rows, the oracle's parser "
+                                  "(Python, per field)")
     kl = datagen.lineitem_keyed(1 << 20, SEED + 31)
 
     def q1_plan_np():
@@ -663,6 +702,7 @@ def main():
         ("select_2^27", "select 2^27 16-byte tuples", lambda: bench_select(rf, ctx)),
         ("mapfilter_2^27", "mapfilter 2^27 lineitem rows", lambda: bench_mapfilter(rf, ctx)),
         ("q1_plan_2^24", "Q1 plan over 2^24 keyed lineitem rows", lambda: bench_q1_plan(rf, ctx)),
+        ("tbl_lineitem_4M", "lineitem.tbl text to columns", lambda: bench_tbl(rf, ctx)),
         ("setop_2^27", "set operations 2^27 + 2^27 u64", lambda: bench_setop(rf, ctx)),
         ("batched_512", "batched 512 join-aggregate instances", lambda: bench_batched(rf, ctx)),
         ("served_mix", "served mixed requests", lambda: bench_served(rf, ctx)),

@@ -3,13 +3,14 @@
 // getline, strtok on the delimiter, atoi per field, one packed tuple per
 // row) as two HBM passes over the text once it is resident:
 //
-//  1. row ends: tiles of 512 threads x 16 bytes count the newlines in each
-//     thread's 16 bytes, scan the counts across the block and take the
+//  1. row ends: tiles of 512 threads x 64 bytes count the newlines in each
+//     thread's 64 bytes, scan the counts across the block and take the
 //     tile's offset from a decoupled look-back, then store the position of
 //     every newline in text order;
-//  2. fields: one thread per row walks its bytes, splitting at the
-//     delimiter, and converts field f by the type spec[f] straight into
-//     column f.
+//  2. fields: a block of 256 rows stages their text (one contiguous span)
+//     in LDS with 16-byte loads; one thread per row then walks its bytes,
+//     splitting at the delimiter, and converts field f by the type spec[f]
+//     straight into column f.
 //
 // Field conversions (C semantics): integers as atoi/atol (leading spaces,
 // a sign, digits up to the first other byte; wrapping), decimals as the
@@ -25,7 +26,7 @@ namespace rfx {
 namespace {
 
 constexpr int kLNT = 512;
-constexpr int kLBytes = 16;
+constexpr int kLBytes = 64;
 constexpr int kLTile = kLNT * kLBytes;
 constexpr int kMaxFields = RFX_TBL_MAX_COLS;
 
@@ -44,26 +45,32 @@ __global__ __launch_bounds__(kLNT) void k_tbl_rows(const uint8_t* __restrict__ t
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint64_t at = static_cast<uint64_t>(tile) * kLTile + static_cast<uint64_t>(tid) * kLBytes;
-  // The thread's 16 bytes as four words; bytes past the end are not newlines.
-  uint32_t word[4] = {0, 0, 0, 0};
+  // The thread's 64 bytes as sixteen words; bytes past the end are not
+  // newlines.
+  uint32_t word[kLBytes / 4];
   if (aligned && at + kLBytes <= bytes) {
-    const uint4 v = *reinterpret_cast<const uint4*>(text + at);
-    word[0] = v.x;
-    word[1] = v.y;
-    word[2] = v.z;
-    word[3] = v.w;
+#pragma unroll
+    for (int j = 0; j < kLBytes / 16; ++j) {
+      const uint4 v = reinterpret_cast<const uint4*>(text + at)[j];
+      word[4 * j] = v.x;
+      word[4 * j + 1] = v.y;
+      word[4 * j + 2] = v.z;
+      word[4 * j + 3] = v.w;
+    }
   } else {
+#pragma unroll
+    for (int j = 0; j < kLBytes / 4; ++j) word[j] = 0;
     for (int b = 0; b < kLBytes; ++b)
       if (at + b < bytes) word[b >> 2] |= static_cast<uint32_t>(text[at + b]) << (8 * (b & 3));
   }
-  uint32_t mask = 0;  // bit b: byte b is a newline
+  uint64_t mask = 0;  // bit b: byte b is a newline
 #pragma unroll
   for (int b = 0; b < kLBytes; ++b) {
     const uint32_t c = (word[b >> 2] >> (8 * (b & 3))) & 0xFF;
-    mask |= static_cast<uint32_t>(c == '\n' && at + b < bytes) << b;
+    mask |= static_cast<uint64_t>(c == '\n' && at + b < bytes) << b;
   }
   uint32_t total = 0;
-  const uint32_t excl = block_excl_scan<kLNT>(static_cast<uint32_t>(__popc(mask)), s_scan, &total);
+  const uint32_t excl = block_excl_scan<kLNT>(static_cast<uint32_t>(__popcll(mask)), s_scan, &total);
   if (w == 0) {
     if (tile == 0) {
       if (lane == 0) {
@@ -82,7 +89,7 @@ __global__ __launch_bounds__(kLNT) void k_tbl_rows(const uint8_t* __restrict__ t
   __syncthreads();
   uint64_t k = s_prefix + excl;
   while (mask) {
-    const int b = __builtin_ctz(mask);
+    const int b = __builtin_ctzll(mask);
     mask &= mask - 1;
     if (k < cap) ends[k] = at + b;
     ++k;
@@ -111,31 +118,29 @@ __device__ const double kPow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  
                                       1e8,  1e9,  1e10, 1e11, 1e12, 1e13, 1e14, 1e15,
                                       1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
 
-// Row r spans [start, end); fields split at `delim`.
-__global__ __launch_bounds__(256) void k_tbl_fields(const uint8_t* __restrict__ text, uint64_t bytes,
-                                                    const uint64_t* __restrict__ ends,
-                                                    uint64_t n_newlines, uint64_t rows,
-                                                    uint8_t delim, int n_fields, TblCols cols,
-                                                    uint64_t* __restrict__ short_rows) {
-  const uint64_t r = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  uint64_t pos = r == 0 ? 0 : ends[r - 1] + 1;
-  uint64_t end = r < n_newlines ? ends[r] : bytes;
-  if (end > pos && text[end - 1] == '\r') --end;
+constexpr int kFNT = 256;            // rows per block
+constexpr int kStage = 48 * 1024;     // staged text bytes per block
+
+// Parse row r ([pos, end) of the text) into the columns; at(q) gives byte q.
+template <class At>
+__device__ __forceinline__ void parse_row(At at, uint64_t r, uint64_t pos, uint64_t end,
+                                          uint8_t delim, int n_fields, const TblCols& cols,
+                                          uint64_t* short_rows) {
+  if (end > pos && at(end - 1) == '\r') --end;
   int f = 0;
   while (f < n_fields) {
     const int t = cols.t[f];
-    // One field: [pos, stop), stop at the delimiter or the row end.
     int64_t iv = 0;
     double dv = 0.0;
     if (t == RFX_TBL_CHAR) {
-      iv = pos < end && text[pos] != delim ? text[pos] : 0;
+      iv = pos < end && at(pos) != delim ? at(pos) : 0;
     } else if (t == RFX_TBL_DATE) {
       // YYYY-MM-DD: each '-'-separated part read as atoi reads it
       // (leading spaces, a '+', digits up to the first other byte).
       int part[3] = {0, 0, 0}, k = 0, state = 0;  // 0 leading, 1 digits, 2 done
-      for (uint64_t q = pos; q < end && text[q] != delim && k < 3; ++q) {
-        const uint8_t c = text[q];
+      for (uint64_t q = pos; q < end && k < 3; ++q) {
+        const uint8_t c = at(q);
+        if (c == delim) break;
         if (c == '-') {
           ++k;
           state = 0;
@@ -149,13 +154,13 @@ __global__ __launch_bounds__(256) void k_tbl_fields(const uint8_t* __restrict__ 
       iv = static_cast<int64_t>(date2int(part[0], part[1], part[2]));
     } else if (t != RFX_TBL_SKIP) {
       uint64_t q = pos;
-      while (q < end && text[q] == ' ') ++q;
+      while (q < end && at(q) == ' ') ++q;
       bool neg = false;
-      if (q < end && (text[q] == '-' || text[q] == '+')) neg = text[q++] == '-';
+      if (q < end && (at(q) == '-' || at(q) == '+')) neg = at(q++) == '-';
       uint64_t m = 0;
       int frac = -1;  // digits after the point
       for (; q < end; ++q) {
-        const uint8_t c = text[q];
+        const uint8_t c = at(q);
         if (c >= '0' && c <= '9') {
           m = m * 10 + (c - '0');
           if (frac >= 0) ++frac;
@@ -173,13 +178,62 @@ __global__ __launch_bounds__(256) void k_tbl_fields(const uint8_t* __restrict__ 
     }
     tbl_store(cols, f, r, iv, dv);
     ++f;
-    while (pos < end && text[pos] != delim) ++pos;
+    while (pos < end && at(pos) != delim) ++pos;
     if (pos >= end) break;
     ++pos;  // past the delimiter
   }
   if (f < n_fields) {
     atomicAdd(reinterpret_cast<unsigned long long*>(short_rows), 1ull);
     for (; f < n_fields; ++f) tbl_store(cols, f, r, 0, 0.0);  // missing fields read 0
+  }
+}
+
+// Rows [256 b, 256 b + 256) of block b.  Their text is one contiguous span:
+// when it fits, the block stages it in LDS with 16-byte loads and every
+// thread parses its row from there; otherwise from global memory.
+__global__ __launch_bounds__(kFNT) void k_tbl_fields(const uint8_t* __restrict__ text,
+                                                     uint64_t bytes, bool aligned,
+                                                     const uint64_t* __restrict__ ends,
+                                                     uint64_t n_newlines, uint64_t rows,
+                                                     uint8_t delim, int n_fields, TblCols cols,
+                                                     uint64_t* __restrict__ short_rows) {
+  __shared__ uint4 s_text[kStage / 16];
+  __shared__ uint64_t s_span[2];
+  const uint64_t r0 = static_cast<uint64_t>(blockIdx.x) * kFNT;
+  if (threadIdx.x == 0) {
+    const uint64_t last = r0 + kFNT - 1 < rows ? r0 + kFNT - 1 : rows - 1;
+    s_span[0] = r0 == 0 ? 0 : ends[r0 - 1] + 1;
+    s_span[1] = last < n_newlines ? ends[last] : bytes;
+  }
+  __syncthreads();
+  const uint64_t base = s_span[0] & ~15ull, stop = s_span[1];
+  const bool staged = aligned && stop - base <= static_cast<uint64_t>(kStage);
+  if (staged) {
+    const uint32_t nvec = static_cast<uint32_t>((stop - base + 15) / 16);
+    for (uint32_t v = threadIdx.x; v < nvec; v += kFNT) {
+      const uint64_t off = base + 16ull * v;
+      if (off + 16 <= bytes) {
+        s_text[v] = *reinterpret_cast<const uint4*>(text + off);
+      } else {
+        uint32_t w[4] = {0, 0, 0, 0};
+        for (int b = 0; off + b < bytes && b < 16; ++b)
+          w[b >> 2] |= static_cast<uint32_t>(text[off + b]) << (8 * (b & 3));
+        s_text[v] = make_uint4(w[0], w[1], w[2], w[3]);
+      }
+    }
+  }
+  __syncthreads();
+  const uint64_t r = r0 + threadIdx.x;
+  if (r >= rows) return;
+  const uint64_t pos = r == 0 ? 0 : ends[r - 1] + 1;
+  const uint64_t end = r < n_newlines ? ends[r] : bytes;
+  if (staged) {
+    const uint8_t* lds = reinterpret_cast<const uint8_t*>(s_text);
+    parse_row([&](uint64_t q) -> uint8_t { return lds[q - base]; }, r, pos, end, delim, n_fields,
+              cols, short_rows);
+  } else {
+    parse_row([&](uint64_t q) -> uint8_t { return text[q]; }, r, pos, end, delim, n_fields, cols,
+              short_rows);
   }
 }
 
@@ -246,9 +300,9 @@ extern "C" int rfx_tbl_parse(rfx_ctx* ctx, const void* text, uint64_t bytes, cha
   const uint64_t todo = n < capacity ? n : capacity;
   if (todo && n_fields) {
     RFX_PROF(ctx, "tbl.fields");
-    k_tbl_fields<<<static_cast<uint32_t>((todo + 255) / 256), 256, 0, ctx->stream>>>(
-        static_cast<const uint8_t*>(text), bytes, ends, newlines, todo,
-        static_cast<uint8_t>(delim), n_fields, cols, short_rows);
+    k_tbl_fields<<<static_cast<uint32_t>((todo + kFNT - 1) / kFNT), kFNT, 0, ctx->stream>>>(
+        static_cast<const uint8_t*>(text), bytes, reinterpret_cast<uintptr_t>(text) % 16 == 0,
+        ends, newlines, todo, static_cast<uint8_t>(delim), n_fields, cols, short_rows);
     RFX_LAUNCHED();
   }
   RFX_HIP(hipStreamSynchronize(ctx->stream));

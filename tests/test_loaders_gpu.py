@@ -52,9 +52,10 @@ def test_lineitem_text_sizes(rf, ctx, n_orders):
 
 
 def test_tile_edge_newlines(rf, ctx):
-    # Newlines at 8191/8192/8193 and at the ends of 16-byte words.
+    # Newlines at 8191/8192/8193, at the ends of 16-byte words, and a block
+    # whose rows span more text than the field pass stages in LDS.
     rows = []
-    for L in (8190, 1, 14, 15, 16, 17, 8191, 31):
+    for L in (8190, 1, 14, 15, 16, 17, 8191, 31, 50_000, 3):  # 50000: past the LDS stage
         rows.append(b"7" * L)
     text = b"\n".join(rows) + b"\n"
     check(rf, ctx, text, ["u32"])
