@@ -599,7 +599,8 @@ class Plan:
         outs = [self._alloc(t, n) for t in out_types]
         cell = self._buf(8)
         x.run(self.ctx, n, outs, cell)
-        kept = rfx.read_u64(cell)
+        # Without a filter every row is kept: no need to wait for the count.
+        kept = n if x.filter is None else rfx.read_u64(cell)
         self._release(cell)
         return Relation(list(zip(outs, out_types)), kept)
 
