@@ -191,6 +191,14 @@ int rfx_join_find_bounds(rfx_ctx* ctx, int32_t* lower_bound,
                          int32_t* left_count, uint64_t* total,
                          const void* left, uint64_t nl, const void* right,
                          uint64_t nr, int key_bytes);
+/* find_bounds over string keys (find_bounds_string, ModernGPUJoin.cu:
+ * 213-312): left/right hold device addresses of NUL-terminated strings,
+ * each side sorted in compare_string2's order (:72-97: bytes compared as
+ * signed char, a proper prefix first; a null address reads as the empty
+ * string).  Outputs as rfx_join_find_bounds; rfx_join_main expands them. */
+int rfx_join_find_bounds_string(rfx_ctx* ctx, int32_t* lower_bound, int32_t* left_count,
+                                uint64_t* total, const uint64_t* left, uint64_t nl,
+                                const uint64_t* right, uint64_t nr);
 /* Load-balanced expansion (redfox::join, ModernGPUJoin.cu:665-733):
  * for g < total, left_idx[g] = i with left_count[i] <= g <
  * left_count[i] + count[i], right_idx[g] = lower_bound[i] + g -
@@ -637,6 +645,11 @@ int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, uint64_t n_r
  *   RFX_TBL_F64   decimal digits[.digits] (mantissa / 10^k, correctly
  *                 rounded below 2^53)   RFX_TBL_DATE  YYYY-MM-DD as i64
  *                 seconds (Date.h)      RFX_TBL_CHAR  the field's first byte
+ *   RFX_TBL_STR   the field's device address (u64), a pointer key for
+ *                 rfx_join_find_bounds_string: the byte ending the field
+ *                 (delimiter, '\r' or newline) is overwritten with NUL, so
+ *                 the text is consumed; a field that runs to the end of the
+ *                 text reads as the null pointer (the empty string)
  * *rows (device u64) gets the row count; rows past `capacity` are counted,
  * not stored.  *short_rows (device u64) counts rows with fewer fields than
  * n_fields (their missing fields read 0).  Synchronises the stream. */
@@ -649,7 +662,8 @@ int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, uint64_t n_r
 #define RFX_TBL_F64 5
 #define RFX_TBL_DATE 6
 #define RFX_TBL_CHAR 7
-int rfx_tbl_parse(rfx_ctx* ctx, const void* text, uint64_t bytes, char delim,
+#define RFX_TBL_STR 8
+int rfx_tbl_parse(rfx_ctx* ctx, void* text, uint64_t bytes, char delim,
                   const int* types, int n_fields, void* const* columns,
                   uint64_t capacity, uint64_t* rows, uint64_t* short_rows);
 

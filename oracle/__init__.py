@@ -453,14 +453,21 @@ def parse_tbl(text: bytes, spec, delim: bytes = b"|"):
     lines = text.split(b"\n")
     if lines and lines[-1] == b"":
         lines.pop()
+    starts, at = [], 0  # offset of every row in the text
+    for line in lines:
+        starts.append(at)
+        at += len(line) + 1
     np_t = {"i32": np.int32, "i64": np.int64, "u32": np.uint32, "u8": np.uint8,
             "f64": np.float64, "date": np.int64, "char": np.uint8}
     out = {f: [] for f, s in enumerate(spec) if s != "skip"}
     short = 0
-    for line in lines:
+    for row, line in enumerate(lines):
         if line.endswith(b"\r"):
             line = line[:-1]
         fields = line.split(delim)
+        field_at = [starts[row]]
+        for fld_ in fields[:-1]:
+            field_at.append(field_at[-1] + len(fld_) + 1)
         if len(fields) < len(spec):
             short += 1
         for f, s in enumerate(spec):
@@ -473,6 +480,10 @@ def parse_tbl(text: bytes, spec, delim: bytes = b"|"):
                 v = _decimal(fld)
             elif s == "char":
                 v = fld[0] if fld else 0
+            elif s == "str":
+                # Offset of the field; None where it has no terminator left
+                # (it runs to the end of the text) or is missing.
+                v = None if fld is None or field_at[f] + len(fld) >= len(text) else field_at[f]
             elif s == "date":
                 parts = (fld.split(b"-") + [b"", b"", b""])[:3]
                 v = date2int(*[_atoi(p) for p in parts])
@@ -483,6 +494,9 @@ def parse_tbl(text: bytes, spec, delim: bytes = b"|"):
     for f, s in enumerate(spec):
         if s == "skip":
             continue
+        if s == "str":
+            cols.append(out[f])
+            continue
         if s == "f64":
             cols.append(np.array(out[f], dtype=np.float64))
         else:
@@ -491,3 +505,41 @@ def parse_tbl(text: bytes, spec, delim: bytes = b"|"):
                         else bits.view(np.int64).astype(np_t[s]) if s != "u32"
                         else bits.astype(np.uint32))
     return cols, short
+
+
+def string_less(a: bytes, b: bytes) -> bool:
+    """compare_string2 (ModernGPUJoin.cu:72-97): bytes as signed char up to
+    the first NUL of either; a proper prefix is less."""
+    i = 0
+    while i < len(a) and i < len(b) and a[i] != 0 and b[i] != 0:
+        x = a[i] - 256 if a[i] >= 128 else a[i]
+        y = b[i] - 256 if b[i] >= 128 else b[i]
+        if x != y:
+            return x < y
+        i += 1
+    end_a = i >= len(a) or a[i] == 0
+    end_b = i >= len(b) or b[i] == 0
+    return end_a and not end_b
+
+
+def string_key(a: bytes):
+    """A sort key giving string_less's order (signed bytes up to a NUL)."""
+    a = a.split(b"\0", 1)[0]
+    return [x - 256 if x >= 128 else x for x in a]
+
+
+def find_bounds_string(left, right):
+    """find_bounds_string over sorted string lists: lower bounds, exclusive
+    prefix of the equal-run counts, and the total (ModernGPUJoin.cu:
+    213-312)."""
+    import bisect
+
+    rk = [string_key(r) for r in right]
+    lb, cnt = [], []
+    for x in left:
+        k = string_key(x)
+        lo = bisect.bisect_left(rk, k)
+        lb.append(lo)
+        cnt.append(bisect.bisect_right(rk, k) - lo)
+    lc = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int64) if cnt else np.zeros(0, np.int64)
+    return np.array(lb, dtype=np.int64), lc, int(sum(cnt))

@@ -110,6 +110,7 @@ __device__ __forceinline__ void tbl_store(const TblCols& c, int f, uint64_t row,
     case RFX_TBL_I64: case RFX_TBL_DATE: static_cast<int64_t*>(p)[row] = iv; break;
     case RFX_TBL_U8: case RFX_TBL_CHAR: static_cast<uint8_t*>(p)[row] = static_cast<uint8_t>(iv); break;
     case RFX_TBL_F64: static_cast<double*>(p)[row] = dv; break;
+    case RFX_TBL_STR: static_cast<int64_t*>(p)[row] = iv; break;
     default: break;
   }
 }
@@ -123,8 +124,9 @@ constexpr int kStage = 48 * 1024;     // staged text bytes per block
 
 // Parse row r ([pos, end) of the text) into the columns; at(q) gives byte q.
 template <class At>
-__device__ __forceinline__ void parse_row(At at, uint64_t r, uint64_t pos, uint64_t end,
-                                          uint8_t delim, int n_fields, const TblCols& cols,
+__device__ __forceinline__ void parse_row(At at, uint8_t* text, uint64_t bytes, uint64_t r,
+                                          uint64_t pos, uint64_t end, uint8_t delim,
+                                          int n_fields, const TblCols& cols,
                                           uint64_t* short_rows) {
   if (end > pos && at(end - 1) == '\r') --end;
   int f = 0;
@@ -152,7 +154,7 @@ __device__ __forceinline__ void parse_row(At at, uint64_t r, uint64_t pos, uint6
         }
       }
       iv = static_cast<int64_t>(date2int(part[0], part[1], part[2]));
-    } else if (t != RFX_TBL_SKIP) {
+    } else if (t != RFX_TBL_SKIP && t != RFX_TBL_STR) {
       uint64_t q = pos;
       while (q < end && at(q) == ' ') ++q;
       bool neg = false;
@@ -176,9 +178,17 @@ __device__ __forceinline__ void parse_row(At at, uint64_t r, uint64_t pos, uint6
         dv = neg ? -mag : mag;
       }
     }
+    const uint64_t start = pos;
+    while (pos < end && at(pos) != delim) ++pos;
+    if (t == RFX_TBL_STR) {
+      // The field's address; its end (delimiter, '\r' or newline) becomes
+      // its terminating NUL.  A field running to the end of the text has
+      // nowhere to end: it reads as the null pointer (the empty string).
+      iv = pos < bytes ? static_cast<int64_t>(reinterpret_cast<uintptr_t>(text + start)) : 0;
+      if (pos < bytes) text[pos] = 0;
+    }
     tbl_store(cols, f, r, iv, dv);
     ++f;
-    while (pos < end && at(pos) != delim) ++pos;
     if (pos >= end) break;
     ++pos;  // past the delimiter
   }
@@ -191,7 +201,7 @@ __device__ __forceinline__ void parse_row(At at, uint64_t r, uint64_t pos, uint6
 // Rows [256 b, 256 b + 256) of block b.  Their text is one contiguous span:
 // when it fits, the block stages it in LDS with 16-byte loads and every
 // thread parses its row from there; otherwise from global memory.
-__global__ __launch_bounds__(kFNT) void k_tbl_fields(const uint8_t* __restrict__ text,
+__global__ __launch_bounds__(kFNT) void k_tbl_fields(uint8_t* __restrict__ text,
                                                      uint64_t bytes, bool aligned,
                                                      const uint64_t* __restrict__ ends,
                                                      uint64_t n_newlines, uint64_t rows,
@@ -229,11 +239,11 @@ __global__ __launch_bounds__(kFNT) void k_tbl_fields(const uint8_t* __restrict__
   const uint64_t end = r < n_newlines ? ends[r] : bytes;
   if (staged) {
     const uint8_t* lds = reinterpret_cast<const uint8_t*>(s_text);
-    parse_row([&](uint64_t q) -> uint8_t { return lds[q - base]; }, r, pos, end, delim, n_fields,
-              cols, short_rows);
+    parse_row([&](uint64_t q) -> uint8_t { return lds[q - base]; }, text, bytes, r, pos, end,
+              delim, n_fields, cols, short_rows);
   } else {
-    parse_row([&](uint64_t q) -> uint8_t { return text[q]; }, r, pos, end, delim, n_fields, cols,
-              short_rows);
+    parse_row([&](uint64_t q) -> uint8_t { return text[q]; }, text, bytes, r, pos, end, delim,
+              n_fields, cols, short_rows);
   }
 }
 
@@ -242,7 +252,7 @@ __global__ __launch_bounds__(kFNT) void k_tbl_fields(const uint8_t* __restrict__
 
 using namespace rfx;
 
-extern "C" int rfx_tbl_parse(rfx_ctx* ctx, const void* text, uint64_t bytes, char delim,
+extern "C" int rfx_tbl_parse(rfx_ctx* ctx, void* text, uint64_t bytes, char delim,
                              const int* types, int n_fields, void* const* columns,
                              uint64_t capacity, uint64_t* rows, uint64_t* short_rows) {
   if (!ctx || !rows || !short_rows || (bytes && !text) || (n_fields && (!types || !columns)))
@@ -253,7 +263,7 @@ extern "C" int rfx_tbl_parse(rfx_ctx* ctx, const void* text, uint64_t bytes, cha
   if (bytes >= (1ull << 40)) return fail(RFX_ERANGE, "rfx_tbl_parse: text too large");
   TblCols cols{};
   for (int f = 0; f < n_fields; ++f) {
-    if (types[f] < RFX_TBL_SKIP || types[f] > RFX_TBL_CHAR)
+    if (types[f] < RFX_TBL_SKIP || types[f] > RFX_TBL_STR)
       return fail(RFX_EINVAL, "rfx_tbl_parse: field %d type %d", f, types[f]);
     if (types[f] != RFX_TBL_SKIP && capacity && !columns[f])
       return fail(RFX_EINVAL, "rfx_tbl_parse: field %d has no column", f);
@@ -301,7 +311,7 @@ extern "C" int rfx_tbl_parse(rfx_ctx* ctx, const void* text, uint64_t bytes, cha
   if (todo && n_fields) {
     RFX_PROF(ctx, "tbl.fields");
     k_tbl_fields<<<static_cast<uint32_t>((todo + kFNT - 1) / kFNT), kFNT, 0, ctx->stream>>>(
-        static_cast<const uint8_t*>(text), bytes, reinterpret_cast<uintptr_t>(text) % 16 == 0,
+        static_cast<uint8_t*>(text), bytes, reinterpret_cast<uintptr_t>(text) % 16 == 0,
         ends, newlines, todo, static_cast<uint8_t>(delim), n_fields, cols, short_rows);
     RFX_LAUNCHED();
   }

@@ -1,0 +1,138 @@
+// strings.hip - sorted search over string keys (find_bounds_string,
+// ModernGPUJoin.cu:213-312) for gfx950.
+//
+// Keys are device addresses of NUL-terminated strings, each side sorted by
+// compare_string2 (ModernGPUJoin.cu:72-97): bytes compared as signed char
+// up to the first NUL of either, a proper prefix ordered first.  One
+// thread per left key finds the first right key not below it and the first
+// right key above it by binary search (every probe a string comparison in
+// global memory); the counts are then turned into their exclusive prefix
+// sums by a tile scan with decoupled look-back, whose last tile writes the
+// total.  The outputs are those of rfx_join_find_bounds, so rfx_join_main
+// expands them into row pairs.
+#include "rfx_device.h"
+#include "rfx_internal.h"
+
+namespace rfx {
+namespace {
+
+// compare_string2: a < b.  A null address reads as the empty string.
+__device__ __forceinline__ bool str_less(uint64_t pa, uint64_t pb) {
+  const int8_t* a = reinterpret_cast<const int8_t*>(pa);
+  const int8_t* b = reinterpret_cast<const int8_t*>(pb);
+  for (uint64_t i = 0;; ++i) {
+    const int8_t x = pa ? a[i] : 0, y = pb ? b[i] : 0;
+    if (x == 0 || y == 0) return x == 0 && y != 0;
+    if (x != y) return x < y;
+  }
+}
+
+constexpr int kSNT = 256;
+
+__global__ __launch_bounds__(kSNT) void k_str_bounds(const uint64_t* __restrict__ left, uint64_t nl,
+                                                     const uint64_t* __restrict__ right,
+                                                     uint64_t nr, int32_t* __restrict__ lower,
+                                                     int32_t* __restrict__ count) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kSNT + threadIdx.x;
+  if (i >= nl) return;
+  const uint64_t key = left[i];
+  uint64_t lo = 0, hi = nr;  // first right key not below key
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (str_less(right[mid], key)) lo = mid + 1;
+    else hi = mid;
+  }
+  uint64_t ub_lo = lo, ub_hi = nr;  // first right key above key
+  while (ub_lo < ub_hi) {
+    const uint64_t mid = (ub_lo + ub_hi) >> 1;
+    if (str_less(key, right[mid])) ub_hi = mid;
+    else ub_lo = mid + 1;
+  }
+  lower[i] = static_cast<int32_t>(lo);
+  count[i] = static_cast<int32_t>(ub_lo - lo);
+}
+
+// count[] -> exclusive prefix sums in place, tiles of 256 x 8; *total gets
+// the sum.
+constexpr int kCVT = 8;
+constexpr int kCTile = kSNT * kCVT;
+
+__global__ __launch_bounds__(kSNT) void k_count_scan(int32_t* __restrict__ count, uint64_t n,
+                                                     uint32_t ntiles, uint64_t* __restrict__ total,
+                                                     uint64_t* __restrict__ status,
+                                                     uint32_t* __restrict__ ticket) {
+  __shared__ uint64_t s_scan[kSNT / kWave + 1];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_prefix;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
+  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t at = static_cast<uint64_t>(tile) * kCTile + static_cast<uint64_t>(tid) * kCVT;
+  uint64_t v[kCVT], sum = 0;
+#pragma unroll
+  for (int j = 0; j < kCVT; ++j) {
+    v[j] = at + j < n ? static_cast<uint32_t>(count[at + j]) : 0u;
+    sum += v[j];
+  }
+  uint64_t tile_sum = 0;
+  const uint64_t excl = block_excl_scan<kSNT>(sum, s_scan, &tile_sum);
+  if (w == 0) {
+    if (tile == 0) {
+      if (lane == 0) {
+        status_store64(status, kFlagIncl64 | tile_sum);
+        s_prefix = 0;
+      }
+    } else {
+      if (lane == 0) status_store64(status + tile, kFlagAgg64 | tile_sum);
+      const uint64_t prev = wave_lookback64(status, tile);
+      if (lane == 0) {
+        status_store64(status + tile, kFlagIncl64 | (prev + tile_sum));
+        s_prefix = prev;
+      }
+    }
+  }
+  __syncthreads();
+  uint64_t run = s_prefix + excl;
+#pragma unroll
+  for (int j = 0; j < kCVT; ++j) {
+    if (at + j < n) count[at + j] = static_cast<int32_t>(run);
+    run += v[j];
+  }
+  if (tile == ntiles - 1 && tid == 0) *total = s_prefix + tile_sum;
+}
+
+}  // namespace
+}  // namespace rfx
+
+using namespace rfx;
+
+extern "C" int rfx_join_find_bounds_string(rfx_ctx* ctx, int32_t* lower_bound, int32_t* left_count,
+                                           uint64_t* total, const uint64_t* left, uint64_t nl,
+                                           const uint64_t* right, uint64_t nr) {
+  if (!ctx || !total || (nl && (!lower_bound || !left_count || !left)) || (nr && !right))
+    return fail(RFX_EINVAL, "rfx_join_find_bounds_string: null argument");
+  if (nl >= (1ull << 31) || nr >= (1ull << 31))
+    return fail(RFX_ERANGE, "rfx_join_find_bounds_string: more than 2^31 - 1 keys");
+  if (nl == 0) return fill_u32(ctx, reinterpret_cast<uint32_t*>(total), 2, 0);
+  const uint64_t ntiles = (nl + kCTile - 1) / kCTile;
+  Carve c(nullptr);
+  c.take<uint64_t>(ntiles);
+  c.take<uint32_t>(1);
+  RFX_TRY(ensure_workspace(ctx, c.off));
+  Carve w(ctx->ws);
+  uint64_t* status = w.take<uint64_t>(ntiles);
+  uint32_t* ticket = w.take<uint32_t>(1);
+  RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(status),
+                   (reinterpret_cast<char*>(ticket + 1) - reinterpret_cast<char*>(status)) / 4, 0));
+  {
+    RFX_PROF(ctx, "join.find_bounds_string");
+    k_str_bounds<<<static_cast<uint32_t>((nl + kSNT - 1) / kSNT), kSNT, 0, ctx->stream>>>(
+        left, nl, right, nr, lower_bound, left_count);
+    RFX_LAUNCHED();
+  }
+  k_count_scan<<<static_cast<uint32_t>(ntiles), kSNT, 0, ctx->stream>>>(
+      left_count, nl, static_cast<uint32_t>(ntiles), total, status, ticket);
+  RFX_LAUNCHED();
+  return RFX_OK;
+}

@@ -14,10 +14,12 @@ import numpy as np
 from . import mapfilter as M
 from . import rfx
 
-TBL = {"skip": 0, "i32": 1, "i64": 2, "u32": 3, "u8": 4, "f64": 5, "date": 6, "char": 7}
-WIDTH = {"skip": 0, "i32": 4, "i64": 8, "u32": 4, "u8": 1, "f64": 8, "date": 8, "char": 1}
+TBL = {"skip": 0, "i32": 1, "i64": 2, "u32": 3, "u8": 4, "f64": 5, "date": 6, "char": 7,
+       "str": 8}
+WIDTH = {"skip": 0, "i32": 4, "i64": 8, "u32": 4, "u8": 1, "f64": 8, "date": 8, "char": 1,
+         "str": 8}
 NUMPY = {"i32": np.int32, "i64": np.int64, "u32": np.uint32, "u8": np.uint8, "f64": np.float64,
-         "date": np.int64, "char": np.uint8}
+         "date": np.int64, "char": np.uint8, "str": np.uint64}
 
 
 def _call(ctx, text, nbytes, delim, spec, columns, capacity):
@@ -35,7 +37,8 @@ def parse_tbl(ctx, text, spec: Sequence[str], delim: str = "|",
               nbytes: Optional[int] = None) -> Tuple[List[Optional[object]], int, int]:
     """Parse rows of `text` (host bytes, or a DeviceBuffer holding `nbytes`)
     into one device column per non-skipped field.  Returns (columns, rows,
-    rows with missing fields)."""
+    rows with missing fields).  "str" fields become device addresses of
+    NUL-terminated strings in the text buffer, which they modify."""
     for s in spec:
         if s not in TBL:
             raise ValueError(f"unknown field type {s!r}")

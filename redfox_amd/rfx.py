@@ -206,6 +206,12 @@ class Context:
             self.handle, lower_bound.ptr, left_count.ptr, total.ptr, left.ptr, nl,
             right.ptr, nr, key_bytes))
 
+    def join_find_bounds_string(self, lower_bound, left_count, total, left, nl, right,
+                                nr) -> None:
+        """left/right: device arrays of string addresses (u64), each sorted."""
+        _check("rfx_join_find_bounds_string", lib().rfx_join_find_bounds_string(
+            self.handle, lower_bound.ptr, left_count.ptr, total.ptr, left.ptr, nl, right.ptr, nr))
+
     def join_main(self, left_idx, right_idx, total, total_cap, lower_bound, left_count,
                   nl) -> None:
         _check("rfx_join_main", lib().rfx_join_main(

@@ -76,3 +76,45 @@ def test_generated_lineitem_text_encodes_its_values():
 def test_unknown_field_type_is_refused():
     with pytest.raises(ValueError):
         loaders.parse_tbl(None, b"1\n", ["int"])
+
+
+def _random_strings(rng, n):
+    alphabet = [b"a", b"b", b"ab", b"\x80", b"\xff", b"z", b""]
+    out = []
+    for _ in range(n):
+        k = int(rng.integers(0, 4))
+        out.append(b"".join(alphabet[int(i)] for i in rng.integers(0, len(alphabet), k)))
+    return out
+
+
+def test_string_order_two_ways():
+    """string_key's order (signed bytes, prefixes first) equals sorting with
+    the compare_string2 transliteration as the comparator."""
+    import functools
+    rng = np.random.default_rng(7)
+    xs = _random_strings(rng, 300) + [b"a\0zz", b"a"]
+    by_key = sorted(xs, key=oracle.string_key)
+    by_cmp = sorted(xs, key=functools.cmp_to_key(
+        lambda a, b: -1 if oracle.string_less(a, b) else 1 if oracle.string_less(b, a) else 0))
+    assert [oracle.string_key(x) for x in by_key] == [oracle.string_key(x) for x in by_cmp]
+    assert oracle.string_less(b"\xff", b"a") and oracle.string_less(b"", b"\x80")
+    assert not oracle.string_less(b"a\0b", b"a") and not oracle.string_less(b"a", b"a\0b")
+
+
+def test_find_bounds_string_against_counting():
+    rng = np.random.default_rng(8)
+    left = sorted(_random_strings(rng, 80), key=oracle.string_key)
+    right = sorted(_random_strings(rng, 120), key=oracle.string_key)
+    lb, lc, total = oracle.find_bounds_string(left, right)
+    run = 0
+    for i, x in enumerate(left):
+        below = sum(oracle.string_less(r, x) for r in right)
+        equal = sum(not oracle.string_less(r, x) and not oracle.string_less(x, r) for r in right)
+        assert lb[i] == below and lc[i] == run
+        run += equal
+    assert total == run
+
+
+def test_string_fields_offsets():
+    cols, short = oracle.parse_tbl(b"ab|cde|1\nx||2\nlast|z", ["str", "str", "i32"])
+    assert cols[0] == [0, 9, 14] and cols[1] == [3, 11, None] and short == 1
