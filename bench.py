@@ -368,6 +368,46 @@ def bench_tbl(rf, ctx, iters=3, n_orders=1 << 20):
             "fields_pass": roofline(len(raw) + (out_bytes + 16) * n, fld_l, fld_ms)}
 
 
+def bench_join_string(rf, ctx, iters=5, nl=1 << 18, nr=1 << 22):
+    """String-key join (find_bounds_string + expansion): 2^18 distinct left
+    keys 'SUPPLIER#%09d' against 2^22 right keys drawn from them (16 per
+    left key), both sorted, as NUL-terminated strings in HBM."""
+    keys = np.array([b"SUPPLIER#%09d" % i for i in range(nl)])
+    width = keys.dtype.itemsize + 1
+    rng = np.random.default_rng(SEED + 50)
+    right_idx = np.sort(rng.integers(0, nl, nr))
+
+    def blob_and_ptrs(idx):
+        mat = np.zeros((len(idx), width), dtype=np.uint8)
+        mat[:, :width - 1] = keys[idx].view(np.uint8).reshape(len(idx), width - 1)
+        buf = rf.DeviceBuffer.from_numpy(mat.reshape(-1))
+        ptrs = rf.DeviceBuffer.from_numpy(buf.ptr + width * np.arange(len(idx), dtype=np.uint64))
+        return buf, ptrs
+    lbuf, lp = blob_and_ptrs(np.arange(nl))
+    rbuf, rp = blob_and_ptrs(right_idx)
+    lb, lc, tot = rf.DeviceBuffer(4 * nl), rf.DeviceBuffer(4 * nl), rf.u64_cell()
+    ctx.join_find_bounds_string(lb, lc, tot, lp, nl, rp, nr)
+    ctx.synchronize()
+    total = rf.read_u64(tot)
+    li, ri = rf.DeviceBuffer(4 * total), rf.DeviceBuffer(4 * total)
+    times = []
+    ctx.profile(True)
+    for _ in range(iters):
+        t = rf.Timer(ctx)
+        t.begin()
+        ctx.join_find_bounds_string(lb, lc, tot, lp, nl, rp, nr)
+        ctx.join_main(li, ri, tot, total, lb, lc, nl)
+        times.append(t.end())
+        t.close()
+    _, fb_ms = ctx.profile_read("join.find_bounds_string")
+    ctx.profile(False)
+    m = median(times)
+    for b in (lbuf, lp, rbuf, rp, lb, lc, tot, li, ri):
+        b.close()
+    return {"nl": nl, "nr": nr, "out_rows": total, "ms": m, "rows_per_s": (nl + nr) / (m * 1e-3),
+            "search_ms": fb_ms / iters}
+
+
 def bench_agg(rf, ctx, iters=5):
     """GROUP BY key: SUM(v), COUNT(*) over 2^28 rows sorted by key (u32
     keys Zipf(1.1) over 2^20 values, f64 values) in one reduce-by-key pass."""
@@ -703,6 +743,7 @@ def main():
         ("mapfilter_2^27", "mapfilter 2^27 lineitem rows", lambda: bench_mapfilter(rf, ctx)),
         ("q1_plan_2^24", "Q1 plan over 2^24 keyed lineitem rows", lambda: bench_q1_plan(rf, ctx)),
         ("tbl_lineitem_4M", "lineitem.tbl text to columns", lambda: bench_tbl(rf, ctx)),
+        ("join_string_2^18x2^22", "string-key join", lambda: bench_join_string(rf, ctx)),
         ("setop_2^27", "set operations 2^27 + 2^27 u64", lambda: bench_setop(rf, ctx)),
         ("batched_512", "batched 512 join-aggregate instances", lambda: bench_batched(rf, ctx)),
         ("served_mix", "served mixed requests", lambda: bench_served(rf, ctx)),
