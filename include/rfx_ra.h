@@ -199,6 +199,12 @@ int rfx_join_find_bounds(rfx_ctx* ctx, int32_t* lower_bound,
 int rfx_join_find_bounds_string(rfx_ctx* ctx, int32_t* lower_bound, int32_t* left_count,
                                 uint64_t* total, const uint64_t* left, uint64_t nl,
                                 const uint64_t* right, uint64_t nr);
+/* Sort n string addresses in place into compare_string2's order, stably
+ * (equal strings keep their order); scratch holds n + 1 u64.  No reference
+ * counterpart takes unsorted string columns: the reference's string keys
+ * arrive sorted.  Synchronises the stream once (the longest string sets
+ * the number of passes). */
+int rfx_sort_strings(rfx_ctx* ctx, uint64_t* addresses, uint64_t* scratch, uint64_t n);
 /* Load-balanced expansion (redfox::join, ModernGPUJoin.cu:665-733):
  * for g < total, left_idx[g] = i with left_count[i] <= g <
  * left_count[i] + count[i], right_idx[g] = lower_bound[i] + g -

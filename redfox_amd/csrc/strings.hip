@@ -102,10 +102,76 @@ __global__ __launch_bounds__(kSNT) void k_count_scan(int32_t* __restrict__ count
   if (tile == ntiles - 1 && tid == 0) *total = s_prefix + tile_sum;
 }
 
+// The order-preserving code of one byte of a string: the terminator 0,
+// then signed chars -128..-1, 1..127 as 1..255 (compare_string2's order).
+__device__ __forceinline__ uint64_t str_code(uint8_t b) {
+  return b == 0 ? 0u : static_cast<uint64_t>((b ^ 0x80u) + (b >= 0x80u ? 1u : 0u));
+}
+
+// The longest string (bytes before the NUL; null addresses are empty)
+// into *longest, which starts at 0.
+__global__ __launch_bounds__(kSNT) void k_str_len(const uint64_t* __restrict__ addr, uint64_t n,
+                                                  uint64_t* __restrict__ longest) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kSNT + threadIdx.x;
+  uint64_t k = 0;
+  if (i < n) {
+    const uint8_t* s = reinterpret_cast<const uint8_t*>(addr[i]);
+    if (s)
+      while (s[k]) ++k;
+  }
+  // One atomic per wave.
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const uint64_t o = __shfl_xor(k, d, kWave);
+    k = o > k ? o : k;
+  }
+  if (lane_id() == 0 && k) atomicMax(reinterpret_cast<unsigned long long*>(longest), k);
+}
+
+// Chunk c of every string as a big-endian key of 8 codes (codes past the
+// end are 0).
+__global__ __launch_bounds__(kSNT) void k_str_chunk(const uint64_t* __restrict__ addr, uint64_t n,
+                                                    uint64_t chunk, uint64_t* __restrict__ key) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kSNT + threadIdx.x;
+  if (i >= n) return;
+  const uint8_t* s = reinterpret_cast<const uint8_t*>(addr[i]);
+  uint64_t k = 0;
+  bool ended = s == nullptr;
+  for (uint64_t q = 0; q < 8 * chunk && !ended; ++q) ended = s[q] == 0;
+  for (int b = 0; b < 8; ++b) {
+    const uint8_t c = ended ? 0 : s[8 * chunk + b];
+    ended = ended || c == 0;
+    k = (k << 8) | str_code(c);
+  }
+  key[i] = k;
+}
+
 }  // namespace
 }  // namespace rfx
 
 using namespace rfx;
+
+extern "C" int rfx_sort_strings(rfx_ctx* ctx, uint64_t* addresses, uint64_t* scratch, uint64_t n) {
+  if (!ctx || (n && (!addresses || !scratch)))
+    return fail(RFX_EINVAL, "rfx_sort_strings: null argument");
+  if (n < 2) return RFX_OK;
+  if (n >= (1ull << 40)) return fail(RFX_ERANGE, "rfx_sort_strings: too many strings");
+  const uint32_t grid = static_cast<uint32_t>((n + kSNT - 1) / kSNT);
+  // The longest string fixes the number of passes (scratch[n] holds it).
+  RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(scratch + n), 2, 0));
+  k_str_len<<<grid, kSNT, 0, ctx->stream>>>(addresses, n, scratch + n);
+  RFX_LAUNCHED();
+  uint64_t longest = 0;
+  RFX_HIP(hipMemcpyAsync(&longest, scratch + n, 8, hipMemcpyDeviceToHost, ctx->stream));
+  RFX_HIP(hipStreamSynchronize(ctx->stream));
+  // LSD over 8-byte chunks, last chunk first: each pass a stable sort of
+  // (chunk key, address) pairs.
+  for (uint64_t chunk = (longest + 7) / 8; chunk-- > 0;) {
+    k_str_chunk<<<grid, kSNT, 0, ctx->stream>>>(addresses, n, chunk, scratch);
+    RFX_LAUNCHED();
+    RFX_TRY(rfx_sort_pairs(ctx, scratch, addresses, n, 8, 64, 8));
+  }
+  return RFX_OK;
+}
 
 extern "C" int rfx_join_find_bounds_string(rfx_ctx* ctx, int32_t* lower_bound, int32_t* left_count,
                                            uint64_t* total, const uint64_t* left, uint64_t nl,

@@ -212,6 +212,12 @@ class Context:
         _check("rfx_join_find_bounds_string", lib().rfx_join_find_bounds_string(
             self.handle, lower_bound.ptr, left_count.ptr, total.ptr, left.ptr, nl, right.ptr, nr))
 
+    def sort_strings(self, addresses, scratch, n) -> None:
+        """Sort n string addresses (u64) in place into compare_string2's
+        order; scratch holds n + 1 u64."""
+        _check("rfx_sort_strings", lib().rfx_sort_strings(self.handle, addresses.ptr,
+                                                          scratch.ptr, n))
+
     def join_main(self, left_idx, right_idx, total, total_cap, lower_bound, left_count,
                   nl) -> None:
         _check("rfx_join_main", lib().rfx_join_main(

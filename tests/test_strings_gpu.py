@@ -79,3 +79,59 @@ def test_find_bounds_string(rf, ctx, nl, nr, seed):
         l_idx, r_idx = li.download(np.int32, total), ri.download(np.int32, total)
         assert all(left[a] == right[b] for a, b in zip(l_idx, r_idx))
         assert np.all(np.diff(l_idx) >= 0)
+
+
+@pytest.mark.parametrize("n,seed", [(1, 1), (2, 2), (1000, 3), (50_000, 4)])
+def test_sort_strings(rf, ctx, n, seed):
+    """Stable sort of string addresses into compare_string2's order: bytes
+    >= 0x80 before ASCII, prefixes first, null addresses as "", lengths up
+    to 40 bytes (5 chunk passes)."""
+    rng = np.random.default_rng(seed)
+    alphabet = [b"a", b"b", b"ab", b"\x80", b"\xff", b"z", b"TRUCK#", b"0123456789"]
+    strings = []
+    for i in range(n):
+        k = int(rng.integers(0, 6))
+        s = b"".join(alphabet[int(j)] for j in rng.integers(0, len(alphabet), k))[:40]
+        strings.append(None if (s == b"" and i % 3 == 0) else s)
+    buf, addr = strings_on_device(rf, strings)
+    scratch = rf.DeviceBuffer(8 * (n + 1))
+    before = addr.download(np.uint64, n)
+    ctx.sort_strings(addr, scratch, n)
+    after = addr.download(np.uint64, n)
+    order = sorted(range(n), key=lambda i: oracle.string_key(strings[i] or b""))  # stable
+    assert after.tolist() == [int(before[i]) for i in order]
+
+
+def test_parse_sort_join_strings(rf, ctx):
+    """Text with string keys on both sides: parse, sort, sorted search,
+    expansion; the pairs are exactly the equal-key row pairs."""
+    rng = np.random.default_rng(12)
+    modes = [b"TRUCK", b"MAIL", b"SHIP", b"AIR", b"REG AIR", b"FOB", b"RAIL", b"\xc3\xa9t\xc3\xa9"]
+    lrows = [modes[int(i)] for i in rng.integers(0, len(modes) - 2, 300)]
+    rrows = [modes[int(i)] for i in rng.integers(0, len(modes), 2000)]
+    sides = []
+    for rows in (lrows, rrows):
+        text = b"".join(b"%d|%s|\n" % (i, s) for i, s in enumerate(rows))
+        raw = np.frombuffer(text, dtype=np.uint8)
+        buf = rf.DeviceBuffer(len(raw) + 16)
+        buf.upload(raw)
+        cols, n, _ = loaders.parse_tbl(ctx, buf, ["u32", "str"], "|", len(raw))
+        ids, addr = cols
+        scratch = rf.DeviceBuffer(8 * (n + 1))
+        # Sort the addresses; the row ids follow through the address order.
+        ptr_before = addr.download(np.uint64, n)
+        ctx.sort_strings(addr, scratch, n)
+        ptr_after = addr.download(np.uint64, n)
+        row_of = {int(p): i for i, p in enumerate(ptr_before)}
+        sides.append((buf, addr, n, [row_of[int(p)] for p in ptr_after]))
+    (_, la, nl, lorder), (_, ra, nr, rorder) = sides
+    lb, lc, tot = rf.DeviceBuffer(4 * nl), rf.DeviceBuffer(4 * nl), rf.u64_cell()
+    ctx.join_find_bounds_string(lb, lc, tot, la, nl, ra, nr)
+    total = rf.read_u64(tot)
+    want = sorted((i, j) for i, a in enumerate(lrows) for j, b in enumerate(rrows) if a == b)
+    assert total == len(want)
+    li, ri = rf.DeviceBuffer(4 * total), rf.DeviceBuffer(4 * total)
+    ctx.join_main(li, ri, tot, total, lb, lc, nl)
+    got = sorted((lorder[a], rorder[b]) for a, b in zip(li.download(np.int32, total),
+                                                       ri.download(np.int32, total)))
+    assert got == want
