@@ -16,14 +16,42 @@
 namespace rfx {
 namespace {
 
-// compare_string2: a < b.  A null address reads as the empty string.
+constexpr uint64_t kLow7 = 0x7F7F7F7F7F7F7F7Full;
+
+// 0x80 in every byte of v that is zero (exact: no false flags).
+__device__ __forceinline__ uint64_t zero_bytes(uint64_t v) {
+  return ~(((v & kLow7) + kLow7) | v | kLow7);
+}
+// 0x80 in every byte of v that is not zero.
+__device__ __forceinline__ uint64_t nonzero_bytes(uint64_t v) {
+  return (((v & kLow7) + kLow7) | v) & ~kLow7;
+}
+
+// The 8 bytes at address p (little-endian), read as aligned words: the
+// word after the first is read only when the first holds no NUL at or
+// after p, so every word read holds a byte of the string (an aligned word
+// does not cross a page).  Bytes after a NUL are don't-cares.
+__device__ __forceinline__ uint64_t load8(uint64_t p) {
+  const uint64_t* w = reinterpret_cast<const uint64_t*>(p & ~7ull);
+  const uint32_t sh = static_cast<uint32_t>(p & 7) * 8;
+  uint64_t v = w[0] >> sh;
+  if (sh && !(zero_bytes(v | (~0ull << (64 - sh))))) v |= w[1] << (64 - sh);
+  return v;
+}
+
+// compare_string2: a < b, 8 bytes per step.  A null address reads as the
+// empty string.
 __device__ __forceinline__ bool str_less(uint64_t pa, uint64_t pb) {
-  const int8_t* a = reinterpret_cast<const int8_t*>(pa);
-  const int8_t* b = reinterpret_cast<const int8_t*>(pb);
-  for (uint64_t i = 0;; ++i) {
-    const int8_t x = pa ? a[i] : 0, y = pb ? b[i] : 0;
-    if (x == 0 || y == 0) return x == 0 && y != 0;
-    if (x != y) return x < y;
+  if (!pa || !pb) return !pa && pb && *reinterpret_cast<const uint8_t*>(pb) != 0;
+  for (;; pa += 8, pb += 8) {
+    const uint64_t va = load8(pa), vb = load8(pb);
+    const uint64_t m = zero_bytes(va) | zero_bytes(vb) | nonzero_bytes(va ^ vb);
+    if (m) {
+      const uint32_t sh = static_cast<uint32_t>(__builtin_ctzll(m)) & ~7u;
+      const uint8_t x = static_cast<uint8_t>(va >> sh), y = static_cast<uint8_t>(vb >> sh);
+      if (x == 0 || y == 0) return x == 0 && y != 0;
+      return static_cast<int8_t>(x) < static_cast<int8_t>(y);
+    }
   }
 }
 
