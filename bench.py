@@ -684,6 +684,13 @@ def cpu_ops_baseline():
     out["tbl_lineitem_4M"] = line(tt.count(b"\n"), timed(lambda: oracle.parse_tbl(tt, loaders.LINEITEM_SPEC)),
                                   "about 2^16 dbgen-style lineitem rows, the oracle's parser "
                                   "(Python, per field)")
+    skeys = [b"SUPPLIER#%09d" % i for i in range(1 << 12)]
+    srng = np.random.default_rng(SEED + 51)
+    sright = [skeys[i] for i in np.sort(srng.integers(0, 1 << 12, 1 << 16))]
+    out["join_string_2^18x2^22"] = line((1 << 12) + (1 << 16), timed(
+        lambda: oracle.find_bounds_string(skeys, sright)),
+        "2^12 left x 2^16 right string keys of the same form, the oracle's sorted search "
+        "(Python bisect)")
     kl = datagen.lineitem_keyed(1 << 20, SEED + 31)
 
     def q1_plan_np():
