@@ -548,7 +548,8 @@ int rfx_harmony_call(const char* kernel_name, const uint64_t* args,
  * A program runs as a kernel compiled for it and its column and output
  * types (hiprtc, on its first call; kept for the process), or through the
  * interpreter kernel while a graph is being captured and the program is
- * not compiled yet, or when compiling fails; "mapfilter.mode" in
+ * not compiled yet, when compiling fails, or once 1024 programs are
+ * compiled (modules stay loaded for the process); "mapfilter.mode" in
  * rfx_ctx_tune forces one.  Both give the same results. */
 #define RFX_MF_MAX_COLS 16
 #define RFX_MF_MAX_OUTS 8

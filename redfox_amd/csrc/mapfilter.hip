@@ -116,7 +116,9 @@ const JitHeader kJitHeaders[] = {
 };
 
 // Kernels compiled so far, by device and source.  Modules stay loaded for
-// the life of the process.
+// the life of the process (a launch may still be running one), so the cache
+// is bounded: past kJitMax programs, new ones are interpreted.
+constexpr size_t kJitMax = 1024;
 std::mutex g_jit_mu;
 std::unordered_map<std::string, hipFunction_t> g_jit_fns;
 
@@ -210,6 +212,9 @@ int jit_function(const std::string& src, hipFunction_t* fn) {
     *fn = it->second;
     return RFX_OK;
   }
+  if (g_jit_fns.size() >= kJitMax)
+    return fail(RFX_ERANGE, "rfx_mapfilter: %zu compiled programs already (the most kept)",
+                g_jit_fns.size());
   constexpr int nh = sizeof(kJitHeaders) / sizeof(kJitHeaders[0]);
   const char* names[nh];
   const char* texts[nh];
