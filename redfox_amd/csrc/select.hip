@@ -18,11 +18,17 @@
 namespace rfx {
 namespace {
 
-constexpr int kFNT = 256;
+#ifndef RFX_SEL_NT
+#define RFX_SEL_NT 512
+#endif
+#ifndef RFX_SEL_VT16
+#define RFX_SEL_VT16 16
+#endif
+constexpr int kFNT = RFX_SEL_NT;
 constexpr int kFNW = kFNT / kWave;
-// Rows per thread: 16, or 8 for 16-byte tuples (register budget).
+// Rows per thread: 16, or fewer for 16-byte tuples (register budget).
 template <class K>
-constexpr int kFVT = sizeof(K) >= 16 ? 8 : 16;
+constexpr int kFVT = sizeof(K) >= 16 ? RFX_SEL_VT16 : 16;
 
 template <class K>
 __device__ __forceinline__ u128 widen(K v) { return to128(v); }
@@ -74,7 +80,8 @@ __global__ __launch_bounds__(kFNT) void k_select(const K* __restrict__ in, uint6
                                                  uint32_t* __restrict__ ticket) {
   constexpr int VT = kFVT<K>;
   constexpr int TILE = kFNT * VT;
-  static_assert(VT * kFNW <= kWave, "one wave scans the step counts");
+  constexpr int kE = VT * kFNW, P = kE > kWave ? kE / kWave : 1;
+  static_assert(kE <= kWave || kE % kWave == 0, "whole runs of step counts per lane");
   __shared__ uint32_t s_cnt[VT * kFNW + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
@@ -107,11 +114,24 @@ __global__ __launch_bounds__(kFNT) void k_select(const K* __restrict__ in, uint6
   __syncthreads();
   // Exclusive scan of the (step, wave) counts in row order (wave 0).
   if (w == 0) {
-    const uint32_t c = lane < VT * kFNW ? s_cnt[lane] : 0u;
-    const uint32_t incl = wave_incl_scan(c);
-    if (lane < VT * kFNW) s_cnt[lane] = incl - c;
+    // Lane l scans entries [l*P, l*P + P) (entry j*kFNW + w: row order).
+    uint32_t c[P], sum = 0;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const uint32_t e = lane * P + q;
+      c[q] = e < kE ? s_cnt[e] : 0u;
+      sum += c[q];
+    }
+    const uint32_t incl = wave_incl_scan(sum);
+    uint32_t run = incl - sum;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const uint32_t e = lane * P + q;
+      if (e < kE) s_cnt[e] = run;
+      run += c[q];
+    }
     const uint32_t tile_kept = __shfl(incl, kWave - 1, kWave);
-    if (lane == 0) s_cnt[VT * kFNW] = tile_kept;
+    if (lane == 0) s_cnt[kE] = tile_kept;
     if (tile == 0) {
       if (lane == 0) {
         status_store64(status, kFlagIncl64 | tile_kept);
@@ -131,7 +151,7 @@ __global__ __launch_bounds__(kFNT) void k_select(const K* __restrict__ in, uint6
 #pragma unroll
   for (int j = 0; j < VT; ++j)
     if ((keep >> j) & 1u) out[prefix + s_cnt[j * kFNW + w] + mbcnt(m[j])] = r[j];
-  if (tile == ntiles - 1 && tid == 0) *n_out = prefix + s_cnt[VT * kFNW];
+  if (tile == ntiles - 1 && tid == 0) *n_out = prefix + s_cnt[kE];
 }
 
 __device__ __forceinline__ u128 load_any(const void* base, uint64_t i, int w) {
