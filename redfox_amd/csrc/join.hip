@@ -451,9 +451,10 @@ template <class K>
 int find_bounds_impl(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, const void* left,
                      uint64_t nl, const void* right, uint64_t nr) {
   if constexpr (sizeof(K) <= 8) {
-    if (ctx->tune_fb_tile == 2048 || (ctx->tune_fb_tile == 0 && kFbNV == 2048))
-      return find_bounds_nv<K, 2048>(ctx, lb, lc, total, left, nl, right, nr);
-    return find_bounds_nv<K, 4096>(ctx, lb, lc, total, left, nl, right, nr);
+    const int nv = ctx->tune_fb_tile ? ctx->tune_fb_tile : kFbNV;
+    if (nv == 2048) return find_bounds_nv<K, 2048>(ctx, lb, lc, total, left, nl, right, nr);
+    if (nv == 4096) return find_bounds_nv<K, 4096>(ctx, lb, lc, total, left, nl, right, nr);
+    return find_bounds_nv<K, kFbNV>(ctx, lb, lc, total, left, nl, right, nr);
   } else {
     return find_bounds_nv<K, 2048>(ctx, lb, lc, total, left, nl, right, nr);
   }
@@ -492,7 +493,8 @@ int expand_impl(rfx_ctx* ctx, int mode, int32_t* li, int32_t* ri, const GatherSp
                 uint64_t nl, const EmitSpec& es = EmitSpec{}) {
   const int nv = ctx->tune_ex_tile ? ctx->tune_ex_tile : kExNV;
   if (nv == 2048) return expand_nv<2048>(ctx, mode, li, ri, gs, total, total_cap, lb, lc, nl, es);
-  return expand_nv<4096>(ctx, mode, li, ri, gs, total, total_cap, lb, lc, nl, es);
+  if (nv == 4096) return expand_nv<4096>(ctx, mode, li, ri, gs, total, total_cap, lb, lc, nl, es);
+  return expand_nv<kExNV>(ctx, mode, li, ri, gs, total, total_cap, lb, lc, nl, es);
 }
 
 int check_gather(const GatherSpec& g) {
