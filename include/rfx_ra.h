@@ -205,6 +205,12 @@ int rfx_join_find_bounds_string(rfx_ctx* ctx, int32_t* lower_bound, int32_t* lef
  * arrive sorted.  Synchronises the stream once (the longest string sets
  * the number of passes). */
 int rfx_sort_strings(rfx_ctx* ctx, uint64_t* addresses, uint64_t* scratch, uint64_t n);
+/* The same order as a permutation: rows (n u32 row numbers below n, e.g.
+ * 0..n-1) is sorted stably by the string of each row, addresses[row]
+ * (left unchanged); scratch holds n + 1 u64.  For string-keyed joins of
+ * relations whose other columns follow the key. */
+int rfx_sort_strings_rows(rfx_ctx* ctx, const uint64_t* addresses, uint32_t* rows,
+                          uint64_t* scratch, uint64_t n);
 /* Load-balanced expansion (redfox::join, ModernGPUJoin.cu:665-733):
  * for g < total, left_idx[g] = i with left_count[i] <= g <
  * left_count[i] + count[i], right_idx[g] = lower_bound[i] + g -

@@ -156,12 +156,13 @@ __global__ __launch_bounds__(kSNT) void k_str_len(const uint64_t* __restrict__ a
 }
 
 // Chunk c of every string as a big-endian key of 8 codes (codes past the
-// end are 0).
-__global__ __launch_bounds__(kSNT) void k_str_chunk(const uint64_t* __restrict__ addr, uint64_t n,
+// end are 0).  With rows, element i is the string of row rows[i].
+__global__ __launch_bounds__(kSNT) void k_str_chunk(const uint64_t* __restrict__ addr,
+                                                    const uint32_t* __restrict__ rows, uint64_t n,
                                                     uint64_t chunk, uint64_t* __restrict__ key) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kSNT + threadIdx.x;
   if (i >= n) return;
-  const uint8_t* s = reinterpret_cast<const uint8_t*>(addr[i]);
+  const uint8_t* s = reinterpret_cast<const uint8_t*>(addr[rows ? rows[i] : i]);
   uint64_t k = 0;
   bool ended = s == nullptr;
   for (uint64_t q = 0; q < 8 * chunk && !ended; ++q) ended = s[q] == 0;
@@ -194,9 +195,32 @@ extern "C" int rfx_sort_strings(rfx_ctx* ctx, uint64_t* addresses, uint64_t* scr
   // LSD over 8-byte chunks, last chunk first: each pass a stable sort of
   // (chunk key, address) pairs.
   for (uint64_t chunk = (longest + 7) / 8; chunk-- > 0;) {
-    k_str_chunk<<<grid, kSNT, 0, ctx->stream>>>(addresses, n, chunk, scratch);
+    k_str_chunk<<<grid, kSNT, 0, ctx->stream>>>(addresses, nullptr, n, chunk, scratch);
     RFX_LAUNCHED();
     RFX_TRY(rfx_sort_pairs(ctx, scratch, addresses, n, 8, 64, 8));
+  }
+  return RFX_OK;
+}
+
+extern "C" int rfx_sort_strings_rows(rfx_ctx* ctx, const uint64_t* addresses, uint32_t* rows,
+                                     uint64_t* scratch, uint64_t n) {
+  if (!ctx || (n && (!addresses || !rows || !scratch)))
+    return fail(RFX_EINVAL, "rfx_sort_strings_rows: null argument");
+  if (n >= (1ull << 32)) return fail(RFX_ERANGE, "rfx_sort_strings_rows: too many rows");
+  if (n < 2) return RFX_OK;
+  const uint32_t grid = static_cast<uint32_t>((n + kSNT - 1) / kSNT);
+  RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(scratch + n), 2, 0));
+  k_str_len<<<grid, kSNT, 0, ctx->stream>>>(addresses, n, scratch + n);
+  RFX_LAUNCHED();
+  uint64_t longest = 0;
+  RFX_HIP(hipMemcpyAsync(&longest, scratch + n, 8, hipMemcpyDeviceToHost, ctx->stream));
+  RFX_HIP(hipStreamSynchronize(ctx->stream));
+  // As rfx_sort_strings, carrying row numbers: each pass keys the rows in
+  // their current order by the chunk of their own string.
+  for (uint64_t chunk = (longest + 7) / 8; chunk-- > 0;) {
+    k_str_chunk<<<grid, kSNT, 0, ctx->stream>>>(addresses, rows, n, chunk, scratch);
+    RFX_LAUNCHED();
+    RFX_TRY(rfx_sort_pairs(ctx, scratch, rows, n, 8, 64, 4));
   }
   return RFX_OK;
 }

@@ -248,7 +248,14 @@ def parse(data: bytes):
 _ENTITY = {"RETURNFLAG": "u8", "LINESTATUS": "u8", "LINENUMBER": "u8", "REGION": "u8",
            "NATION": "u8", "SUPPLIER": "u16", "ORDER": "u32", "PART": "u32", "CUSTOMER": "u32"}
 _NP = {"u8": np.uint8, "u16": np.uint16, "u32": np.uint32, "u64": np.uint64, "i8": np.int8,
-       "i16": np.int16, "i32": np.int32, "i64": np.int64, "f32": np.float32, "f64": np.float64}
+       "i16": np.int16, "i32": np.int32, "i64": np.int64, "f32": np.float32, "f64": np.float64,
+       "str": np.uint64}
+
+
+def _mft(t: str) -> str:
+    """The MAPFILTER element type of a relation column type (string columns
+    are carried as their 64-bit addresses)."""
+    return "u64" if t == "str" else t
 _BITS = {"u8": 8, "u16": 16, "u32": 32, "u64": 64, "i8": 8, "i16": 16, "i32": 32, "i64": 64}
 
 
@@ -272,8 +279,10 @@ def column_type(t) -> str:
         if cap not in (32, 64):
             raise ValueError(f"invalid float size {cap}")
         return "f" + str(cap)
-    raise ValueError(f"primitive type {k} is not supported (strings, decimals and "
-                     "blobs are outside the numeric operators)")
+    if k == 9:  # STRING: the address of a NUL-terminated string in device memory
+        return "str"
+    raise ValueError(f"primitive type {k} is not supported (decimals and blobs are "
+                     "outside the numeric operators)")
 
 
 def _num_type(nt) -> str:
@@ -370,8 +379,10 @@ def lower_exp(exp, col) -> Tuple[object, str]:
         return M.Const(v), t
     if tag == 3:  # ARITHEXP
         t = _num_type(exp.arithexp.domain)
-        a, _ = lower_exp(exp.arithexp.exp1, col)
-        b, _ = lower_exp(exp.arithexp.exp2, col)
+        a, ta = lower_exp(exp.arithexp.exp1, col)
+        b, tb = lower_exp(exp.arithexp.exp2, col)
+        if "str" in (ta, tb):
+            raise ValueError("arithmetic on string columns is not supported")
         a, b = _as(a, t), _as(b, t)
         op = exp.arithexp.op
         if op == 4 and _kind(t) == "f":
@@ -422,8 +433,10 @@ def lower_compare(cmp, col):
     if cmp.tag == 3:
         return None
     if cmp.tag == 0:
-        a, _ = lower_exp(cmp.comp.op1, col)
-        b, _ = lower_exp(cmp.comp.op2, col)
+        a, ta = lower_exp(cmp.comp.op1, col)
+        b, tb = lower_exp(cmp.comp.op2, col)
+        if "str" in (ta, tb):
+            raise ValueError("comparisons of string columns are not supported")
         op = _CMP[cmp.comp.comparison]
         return a.eq(b) if op == "eq" else a.ne(b) if op == "ne" else \
             {"lt": a < b, "le": a <= b, "gt": a > b, "ge": a >= b}[op]
@@ -528,7 +541,8 @@ class Plan:
                 b.upload(a)
             cols.append((b, t))
             host.append(a)
-        self._replace(name, Relation(cols, n, sorted_prefix(host)))
+        lead = types.index("str") if "str" in types else len(types)
+        self._replace(name, Relation(cols, n, sorted_prefix(host[:lead])))
 
     def set_columns(self, name: str, columns: Sequence[Tuple[object, str]], n: int,
                     sorted_by: int = 0) -> None:
@@ -594,8 +608,9 @@ class Plan:
             raise ValueError(f"{len(columns)} columns in one pass (at most {M.MAX_COLS})")
         if len(out_types) > M.MAX_OUTS:
             return self._mapfilter_wide(columns, n, build, out_types)
-        x = M.MapFilter(columns)
+        x = M.MapFilter([(c[0], _mft(c[1])) + tuple(c[2:]) for c in columns])
         build(x)
+        x.outputs = [(e, _mft(t)) for e, t in x.outputs]
         outs = [self._alloc(t, n) for t in out_types]
         cell = self._buf(8)
         x.run(self.ctx, n, outs, cell)
@@ -666,7 +681,7 @@ class Plan:
                 x.emit(_as(v, t), t)
 
         # The output types come from the expressions: lower once to learn them.
-        probe = M.MapFilter([(b, t) for b, t in src.cols])
+        probe = M.MapFilter([(b, _mft(t)) for b, t in src.cols])
         types = [lower_exp(e, lambda i: (probe.cols[i], src.types[i]))[1] for e in node.reordering]
         if src.n == 0:
             return self._empty(types)
@@ -683,17 +698,26 @@ class Plan:
             raise ValueError("joined key columns differ in type")
         comb = A.types + B.types[k:]
         if args:
-            probe = M.MapFilter([(None, t) for t in comb])
+            probe = M.MapFilter([(None, _mft(t)) for t in comb])
             out_types = [lower_exp(e, lambda i: (probe.cols[i], comb[i]))[1] for e in args]
         else:
             out_types = comb
         if A.n == 0 or B.n == 0:
             return self._empty(out_types)
-        ka, pa, kt, _, _ = self._pack(A, range(k))
-        kb, pb, _, _, _ = self._pack(B, range(k))
-        kbytes = 4 if kt == "u32" else 8
+        strings = "str" in A.types[:k]
+        if strings and k != 1:
+            raise ValueError("a join on a string key takes that one key column")
         lb, lc, tot = self._alloc("i32", A.n), self._alloc("i32", A.n), self._buf(8)
-        self.ctx.join_find_bounds(lb, lc, tot, ka, A.n, kb, B.n, kbytes)
+        if strings:
+            # Rows of each side in string order, their addresses in that
+            # order, then the sorted search on the strings.
+            ka, pa = self._string_order(A)
+            kb, pb = self._string_order(B)
+            self.ctx.join_find_bounds_string(lb, lc, tot, ka, A.n, kb, B.n)
+        else:
+            ka, pa, kt, _, _ = self._pack(A, range(k))
+            kb, pb, _, _, _ = self._pack(B, range(k))
+            self.ctx.join_find_bounds(lb, lc, tot, ka, A.n, kb, B.n, 4 if kt == "u32" else 8)
         total = rfx.read_u64(tot)
         if total >= 1 << 31:
             self._release(ka, kb, pa, pb, lb, lc, tot)
@@ -731,10 +755,22 @@ class Plan:
         out = self._mapfilter(cols, total, build, out_types)
         self._release(ka, kb, pa, pb, lb, lc, tot, li, ri)
         # Pairs come in key order, left rows in their order within a key:
-        # sorted by A's sorted columns where A needed no sort, else by the key.
-        s_out = A.sorted_by if pa is None else k
+        # sorted by A's sorted columns where A needed no sort, else by the key
+        # (string order is not the addresses' order: no claim).
+        s_out = 0 if strings else A.sorted_by if pa is None else k
         out.sorted_by = _identity_prefix(args, s_out) if args else s_out
         return out
+
+    def _string_order(self, rel: Relation):
+        """(addresses of column 0 in string order, the row permutation)."""
+        n = rel.n
+        perm, scratch = self._alloc("u32", n), self._alloc("u64", n + 1)
+        self.ctx.iota_u32(perm, n)
+        self.ctx.sort_strings_rows(rel.cols[0][0], perm, scratch, n)
+        self._release(scratch)
+        keys = self._mapfilter([(perm, "u32"), (rel.cols[0][0], "u64", n)], n,
+                               lambda x: x.emit(x.gather(1, x.cols[0]), "u64"), ["u64"]).cols[0][0]
+        return keys, perm
 
     def agg(self, node) -> Relation:
         src = self.rel[node.srcA]
@@ -923,10 +959,10 @@ class Builder:
     """Builds a GPUGraph with the plan schema's messages.
 
     Types are written "entity:NAME", "int64", "uint32", "float64",
-    "datetime" or "bool"; expressions with idx(), const_*(), arith(),
+    "datetime", "bool" or "string"; expressions with idx(), const_*(), arith(),
     call(); predicates with comp(), and_(), always()."""
 
-    _PRIM = {"bool": 0, "datetime": 5, "int": 6, "uint": 7, "float": 8}
+    _PRIM = {"bool": 0, "datetime": 5, "int": 6, "uint": 7, "float": 8, "string": 9}
 
     def __init__(self, name: str):
         self.m = messages()

@@ -218,6 +218,12 @@ class Context:
         _check("rfx_sort_strings", lib().rfx_sort_strings(self.handle, addresses.ptr,
                                                           scratch.ptr, n))
 
+    def sort_strings_rows(self, addresses, rows, scratch, n) -> None:
+        """Sort n row numbers (u32) stably by the string each row's address
+        points to; scratch holds n + 1 u64."""
+        _check("rfx_sort_strings_rows", lib().rfx_sort_strings_rows(
+            self.handle, addresses.ptr, rows.ptr, scratch.ptr, n))
+
     def join_main(self, left_idx, right_idx, total, total_cap, lower_bound, left_count,
                   nl) -> None:
         _check("rfx_join_main", lib().rfx_join_main(

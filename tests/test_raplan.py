@@ -140,9 +140,13 @@ def test_wide_keys_and_unsupported_types_are_refused():
         raplan.key_layout(["f64"])
     m = raplan.messages()
     t = m["Type"](kind=0)
-    t.primitive.kind = 9  # STRING
+    t.primitive.kind = 10  # DECIMAL
     with pytest.raises(ValueError):
         raplan.column_type(t)
+    t.primitive.kind = 9  # STRING: device addresses of NUL-terminated strings
+    assert raplan.column_type(t) == "str"
+    with pytest.raises(ValueError):
+        raplan.key_layout(["str"])
     t = m["Type"](kind=1)
     t.unary.name = "PLANET"
     with pytest.raises(ValueError):
@@ -165,3 +169,13 @@ def test_q1_arithmetic_lowering_matches_the_expression():
     v, t = raplan.lower_exp(sub, lambda i: (M.Column(0, "i64"), "i64"))
     got, _ = oracle.mapfilter_eval(v.tree(), [np.array([7776000], dtype=np.int64)])
     assert t == "i64" and got.tolist() == [912470400 - 7776000]
+
+
+def test_string_predicates_are_refused():
+    b = raplan.Builder("x")
+    x = M.MapFilter([(_Buf(), "u64")])
+    col = lambda i: (x.cols[i], "str")  # noqa: E731
+    with pytest.raises(ValueError, match="string"):
+        raplan.lower_compare(b.comp("eq", b.idx(0, "string"), b.const_int(1)), col)
+    with pytest.raises(ValueError, match="string"):
+        raplan.lower_exp(b.arith("add", b.idx(0, "string"), b.const_int(1), "int"), col)

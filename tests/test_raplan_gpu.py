@@ -126,3 +126,39 @@ def test_set_operations_single_product_minmax(ctx):
     assert rel("lo") == [(k, fv[fk == k].min()) for k in ks]
     assert rel("hi") == [(k, fv[fk == k].max()) for k in ks]
     assert rel("neg") == [(x, y) for x, y in A if y < 0 and x != 3]
+
+
+def test_string_key_join_plan(rf, ctx):
+    """Two relations with a STRING key, loaded from text: a key join (rows
+    of each side ordered by their strings, sorted search on the strings),
+    then a projection of the joined fields; every equal-name pair once."""
+    from redfox_amd import loaders
+
+    rng = np.random.default_rng(21)
+    names = [b"FRANCE", b"GERMANY", b"CHINA", b"PERU", b"EGYPT", b"\xc3\x89IRE", b"KENYA"]
+    a_rows = [(names[int(i)], int(v)) for i, v in zip(rng.integers(0, 6, 400),
+                                                       rng.integers(-1000, 1000, 400))]
+    b_rows = [(names[int(i)], float(v)) for i, v in zip(rng.integers(1, 7, 900),
+                                                         rng.integers(0, 10 ** 6, 900))]
+    b = raplan.Builder("strings")
+    b.var("A", ["string"], ["int32"]).var("B", ["string"], ["float64"])
+    b.var("J", ["string"], ["int32", "float64"]).var("P", fields=["int32", "float64"])
+    s = b.sequence(0)
+    s.join("J", 1, "A", "B")
+    s.mapfilter("P", "J", 3, b.always(), [b.idx(1, "int32"), b.idx(2, "float64")])
+    s.halt()
+    p = raplan.Plan(ctx, b.entry_exit(0, 0).bytes())
+    keep = []
+    for name, rows, typ in (("A", a_rows, "i32"), ("B", b_rows, "f64")):
+        text = b"".join(b"%s|%s|\n" % (k, str(v).encode()) for k, v in rows)
+        raw = np.frombuffer(text, dtype=np.uint8)
+        buf = rf.DeviceBuffer(len(raw) + 16)
+        buf.upload(raw)
+        (addr, val), n, _ = loaders.parse_tbl(ctx, buf, ["str", typ], "|", len(raw))
+        p.set_columns(name, [(addr, "str"), (val, "i32" if typ == "i32" else "f64")], n)
+        keep.append(buf)  # the strings live in the text buffers
+    p.run()
+    x, y = p.get("P")
+    want = sorted((va, vb) for ka, va in a_rows for kb, vb in b_rows if ka == kb)
+    assert sorted(zip(x.tolist(), y.tolist())) == want
+    assert p.rows("J") == len(want)
