@@ -617,6 +617,12 @@ int rfx_harmony_call(const char* kernel_name, const uint64_t* args,
 #define RFX_MF_ADD_MONTH 60 /* date R[a] plus imm months (AddMonth, :2053-2090) */
 #define RFX_MF_ADD_YEAR 61  /* date R[a] plus imm years (AddYear) */
 #define RFX_MF_PART_YEAR 62 /* year of date R[a] (PartYear, :2091-2123) */
+#define RFX_MF_SCHUNK 63   /* chunk imm of the string at address R[a] (0: the empty
+                              string): 8 bytes as big-endian codes, the NUL and
+                              what follows it 0, other bytes as signed char
+                              -128..127 -> 1..255, so chunks compare (unsigned)
+                              in compare_string2's order.  R[a] must be 0 or the
+                              address of a NUL-terminated string in device memory */
 #define RFX_MF_OUT 70      /* output column b at the row's rank = R[a], converted to
                               the column type (integers truncated, F32 rounded) */
 

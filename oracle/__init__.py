@@ -327,6 +327,18 @@ def mapfilter_eval(tree, cols):
             sub = list(cols)
             sub[tree[1]] = picked
             return mapfilter_eval(("col", tree[1]), sub)
+        if tag == "schunk":
+            # Chunk k of each row's string: an object column of bytes (None:
+            # the empty string) read through ("col", i).
+            if tree[1][0] != "col" or cols[tree[1][1]].dtype != object:
+                raise ValueError("schunk reads a column of byte strings")
+            k = tree[2]
+            out = []
+            for s in cols[tree[1][1]]:
+                s = (s or b"").split(b"\0", 1)[0]
+                codes = [(b ^ 0x80) + (1 if b >= 0x80 else 0) for b in s] + [0] * (8 * k + 8)
+                out.append(int.from_bytes(bytes(codes[8 * k:8 * k + 8]), "big"))
+            return np.array(out, dtype=np.uint64).view(np.int64), "u"
         if tag == "int":
             return np.array(tree[1] & 0xFFFFFFFFFFFFFFFF, dtype=np.uint64).view(np.int64), "i"
         if tag == "float":

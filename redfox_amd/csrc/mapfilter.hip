@@ -84,7 +84,7 @@ bool valid_code(uint32_t c) {
   return c <= RFX_MF_LDX || (c >= RFX_MF_IADD && c <= RFX_MF_IMOD) ||
          (c >= RFX_MF_FADD && c <= RFX_MF_FDIV) || (c >= RFX_MF_I2F && c <= RFX_MF_F2I) ||
          (c >= RFX_MF_ILT && c <= RFX_MF_FNE) || (c >= RFX_MF_AND && c <= RFX_MF_NOT) ||
-         (c >= RFX_MF_ADD_MONTH && c <= RFX_MF_PART_YEAR) || c == RFX_MF_OUT;
+         (c >= RFX_MF_ADD_MONTH && c <= RFX_MF_SCHUNK) || c == RFX_MF_OUT;
 }
 
 // ---------------------------------------------------------------------------
@@ -319,6 +319,9 @@ extern "C" int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, u
       if (filter || op.b >= n_outs) return fail(RFX_EINVAL, "rfx_mapfilter: op %u writes output %u", pc, op.b);
     } else if (!valid_code(op.code)) {
       return fail(RFX_EINVAL, "rfx_mapfilter: op %u code %u", pc, op.code);
+    } else if (op.code == RFX_MF_SCHUNK && op.imm > 4095) {
+      return fail(RFX_EINVAL, "rfx_mapfilter: op %u string chunk %llu", pc,
+                  static_cast<unsigned long long>(op.imm));
     }
     if (op.d >= RFX_MF_REGS || op.b >= RFX_MF_REGS ||
         (op.code != RFX_MF_LDC && op.code != RFX_MF_LDX && op.a >= RFX_MF_REGS))

@@ -147,6 +147,22 @@ __device__ __forceinline__ void wrapdate(int& year, int& month) {
   }
 }
 
+// SCHUNK: chunk k of the NUL-terminated string at address p as 8
+// order-preserving byte codes, first byte most significant (the terminator
+// and everything after it 0; signed chars -128..-1, 1..127 as 1..255).
+__device__ inline uint64_t mf_schunk(uint64_t p, uint64_t k) {
+  const uint8_t* s = reinterpret_cast<const uint8_t*>(p);
+  bool ended = s == nullptr;
+  for (uint64_t q = 0; q < 8 * k && !ended; ++q) ended = s[q] == 0;
+  uint64_t v = 0;
+  for (int b = 0; b < 8; ++b) {
+    const uint8_t c = ended ? 0 : s[8 * k + b];
+    ended = ended || c == 0;
+    v = (v << 8) | (c == 0 ? 0u : static_cast<uint64_t>((c ^ 0x80u) + (c >= 0x80u ? 1u : 0u)));
+  }
+  return v;
+}
+
 // The value of one op that is neither a column load (LDC) nor a store
 // (OUT), from its operand registers a and b and its immediate.  With a
 // constant code the switch folds to the one operation.
@@ -180,6 +196,7 @@ __device__ __forceinline__ uint64_t mf_apply(uint32_t code, uint64_t a, uint64_t
     case RFX_MF_AND: return static_cast<uint64_t>((a != 0) & (b != 0));
     case RFX_MF_OR: return static_cast<uint64_t>((a != 0) | (b != 0));
     case RFX_MF_NOT: return a == 0;
+    case RFX_MF_SCHUNK: return mf_schunk(a, imm);
     case RFX_MF_ADD_MONTH:
     case RFX_MF_ADD_YEAR:
     case RFX_MF_PART_YEAR: {

@@ -27,7 +27,7 @@ TYPES = {"u8": 1, "u16": 2, "u32": 3, "u64": 4, "i8": 5, "i16": 6, "i32": 7, "i6
 OP = dict(NOP=0, LDC=1, LDI=2, MOV=3, LDX=4, IADD=10, ISUB=11, IMUL=12, IDIV=13, IMOD=14, FADD=20,
           FSUB=21, FMUL=22, FDIV=23, I2F=30, U2F=31, F2I=32, ILT=40, ILE=41, IEQ=42, INE=43,
           ULT=44, ULE=45, FLT=46, FLE=47, FEQ=48, FNE=49, AND=50, OR=51, NOT=52, ADD_MONTH=60,
-          ADD_YEAR=61, PART_YEAR=62, OUT=70)
+          ADD_YEAR=61, PART_YEAR=62, SCHUNK=63, OUT=70)
 MAX_OPS, MAX_COLS, MAX_OUTS, REGS = 96, 16, 8, 16
 
 
@@ -167,12 +167,12 @@ class Unary(Expr):
     def __init__(self, op, x, k=0):
         self.op, self.x, self.k = op, x, k
         self.type = {"not": "i", "i2f": "f", "f2i": "i", "add_month": "i", "add_year": "i",
-                     "part_year": "i"}[op]
-        if op in ("add_month", "add_year", "part_year", "not") and x.type == "f":
+                     "part_year": "i", "schunk": "u"}[op]
+        if op in ("add_month", "add_year", "part_year", "not", "schunk") and x.type == "f":
             raise ValueError(f"{op} needs an integer operand")
 
     def tree(self):
-        if self.op in ("add_month", "add_year"):
+        if self.op in ("add_month", "add_year", "schunk"):
             return (self.op, self.x.tree(), self.k)
         return (self.op, self.x.tree())
 
@@ -182,6 +182,42 @@ def to_int(e) -> Expr: return Unary("f2i", lift(e)) if lift(e).type == "f" else 
 def add_month(e, k: int) -> Expr: return Unary("add_month", lift(e), int(k))
 def add_year(e, k: int) -> Expr: return Unary("add_year", lift(e), int(k))
 def part_year(e) -> Expr: return Unary("part_year", lift(e))
+def str_chunk(e, k: int) -> Expr: return Unary("schunk", lift(e), int(k))
+
+
+def _const_chunks(value: bytes, n: int):
+    """The first n chunks of a constant string as SCHUNK computes them."""
+    codes = [0 if b == 0 else (b ^ 0x80) + (1 if b >= 0x80 else 0)
+             for b in value.split(b"\0", 1)[0]]
+    codes += [0] * (8 * n - len(codes))
+    return [int.from_bytes(bytes(codes[8 * k:8 * k + 8]), "big") for k in range(n)]
+
+
+def str_compare(e, value: bytes, op: str) -> Expr:
+    """Compare the string at address e with a constant string (op: eq, ne,
+    lt, le, gt, ge) in compare_string2's order, chunk by chunk: chunks up to
+    the one holding the constant's terminator decide."""
+    value = value.split(b"\0", 1)[0]
+    n = len(value) // 8 + 1
+    c = _const_chunks(value, n)
+    x = [str_chunk(e, k) for k in range(n)]
+    if op in ("eq", "ne"):
+        r = x[0].eq(c[0])
+        for k in range(1, n):
+            r = r & x[k].eq(c[k])
+        return r if op == "eq" else ~r
+    strict = op in ("lt", "gt")
+    less = op in ("lt", "le")
+    # Lexicographic: decided at the first chunk that differs.
+    r = None
+    for k in reversed(range(n)):
+        here = (x[k] < c[k]) if less else (x[k] > c[k])
+        if k == n - 1:
+            tail = here if strict else (here | x[k].eq(c[k]))
+        else:
+            tail = here | (x[k].eq(c[k]) & r)
+        r = tail
+    return r
 
 
 class _Gen:
@@ -255,6 +291,8 @@ class _Gen:
                 self.conv(r, e.x.type, "f")
             elif e.op == "f2i":
                 self.conv(r, "f", "i")
+            elif e.op == "schunk":
+                self.emit("SCHUNK", r, r, 0, e.k)
             else:
                 self.emit(e.op.upper(), r, r, 0, e.k)
         else:

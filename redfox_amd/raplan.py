@@ -433,10 +433,23 @@ def lower_compare(cmp, col):
     if cmp.tag == 3:
         return None
     if cmp.tag == 0:
-        a, ta = lower_exp(cmp.comp.op1, col)
-        b, tb = lower_exp(cmp.comp.op2, col)
+        op = _CMP[cmp.comp.comparison]
+        e1, e2 = cmp.comp.op1, cmp.comp.op2
+        s1 = e2.tag == 0 and e2.constExp.literal.kind == 3  # a string constant
+        s2 = e1.tag == 0 and e1.constExp.literal.kind == 3
+        if s1 or s2:
+            # A string column against a string constant: chunk comparisons.
+            ecol, econst = (e1, e2) if s1 else (e2, e1)
+            a, ta = lower_exp(ecol, col)
+            if ta != "str":
+                raise ValueError("a string constant compares with a string column")
+            if not s1:  # constant first: mirror the comparison
+                op = {"lt": "gt", "le": "ge", "gt": "lt", "ge": "le"}.get(op, op)
+            return M.str_compare(a, econst.constExp.literal.string_constant.value.encode(), op)
+        a, ta = lower_exp(e1, col)
+        b, tb = lower_exp(e2, col)
         if "str" in (ta, tb):
-            raise ValueError("comparisons of string columns are not supported")
+            raise ValueError("string columns compare only with string constants")
         op = _CMP[cmp.comp.comparison]
         return a.eq(b) if op == "eq" else a.ne(b) if op == "ne" else \
             {"lt": a < b, "le": a <= b, "gt": a > b, "ge": a >= b}[op]

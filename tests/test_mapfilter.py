@@ -204,3 +204,41 @@ def test_struct_layouts_match_the_header():
     assert ctypes.sizeof(M.Op) == 16 and ctypes.sizeof(M.Col) == 24
     assert ctypes.sizeof(M.Out) == 16 and ctypes.sizeof(M.Program) == 16 + 16 * M.MAX_OPS
     assert M.Col.rows.offset == 16
+
+
+def test_string_compare_against_compare_string2():
+    """str_compare (SCHUNK chunks, unsigned chunk order) evaluated by the
+    oracle equals the compare_string2 transliteration for every operator,
+    constants of 0 to 17 bytes, bytes >= 0x80 and prefixes."""
+    rng = np.random.default_rng(1)
+    alpha = [b"A", b"B", b"AB", b"\x80", b"\xff", b"Z", b"MAIL", b"TRUCK", b"AUTOMOBILE"]
+    strs = [b"".join(alpha[int(i)] for i in rng.integers(0, len(alpha), int(rng.integers(0, 4))))
+            for _ in range(400)] + [None]
+    col = np.empty(len(strs), dtype=object)
+    col[:] = strs
+    lt = oracle.string_less
+    for const in [b"", b"MAIL", b"AUTOMOBI", b"AUTOMOBILE", b"\x80AB", b"Z" * 17]:
+        for op in ("eq", "ne", "lt", "le", "gt", "ge"):
+            x = M.MapFilter([(_Buf(), "u64")])
+            v, _ = oracle.mapfilter_eval(M.str_compare(x.cols[0], const, op).tree(), [col])
+            want = []
+            for s in strs:
+                s = s or b""
+                want.append(int({"eq": not lt(s, const) and not lt(const, s),
+                                 "ne": lt(s, const) or lt(const, s), "lt": lt(s, const),
+                                 "le": not lt(const, s), "gt": lt(const, s),
+                                 "ge": not lt(s, const)}[op]))
+            assert v.tolist() == want, (const, op)
+
+
+def test_schunk_chunk_index_is_bounded():
+    import redfox_amd
+    L = redfox_amd.lib()
+    prog = M.Program()
+    prog.n_ops = 1
+    prog.ops[0].code, prog.ops[0].imm = M.OP["SCHUNK"], 5000
+    cols = (M.Col * 1)()
+    cols[0].data, cols[0].type = 1, M.TYPES["u64"]
+    assert L.rfx_mapfilter(ctypes.c_void_p(8), cols, 1, 10, ctypes.byref(prog), None, 0,
+                           ctypes.c_void_p(16)) == -1
+    assert b"chunk" in L.rfx_last_error()

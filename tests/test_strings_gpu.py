@@ -135,3 +135,71 @@ def test_parse_sort_join_strings(rf, ctx):
     got = sorted((lorder[a], rorder[b]) for a, b in zip(li.download(np.int32, total),
                                                        ri.download(np.int32, total)))
     assert got == want
+
+
+@pytest.fixture(params=[1, 2], ids=["compiled", "interpreted"])
+def mctx(ctx, request):
+    ctx.tune("mapfilter.mode", request.param)
+    yield ctx
+    ctx.tune("mapfilter.mode", 0)
+
+
+def test_mapfilter_string_predicates(rf, mctx):
+    """MAPFILTER filters on string columns against constants (SCHUNK chunk
+    comparisons), in both kernels, against the oracle on the same bytes."""
+    from redfox_amd import mapfilter as M
+
+    rng = np.random.default_rng(31)
+    modes = [b"TRUCK", b"MAIL", b"SHIP", b"AIR", b"REG AIR", b"FOB", b"RAIL", b"AUTOMOBILE",
+             b"AUTOMOBI", b"\xc3\xa9t\xc3\xa9", b""]
+    rows = [modes[int(i)] for i in rng.integers(0, len(modes), 30_000)]
+    text = b"".join(b"%d|%s|\n" % (i, s) for i, s in enumerate(rows))
+    raw = np.frombuffer(text, dtype=np.uint8)
+    buf = rf.DeviceBuffer(len(raw) + 16)
+    buf.upload(raw)
+    (ids, addr), n, _ = loaders.parse_tbl(mctx, buf, ["u32", "str"], "|", len(raw))
+    col = np.empty(n, dtype=object)
+    col[:] = rows
+    for const, op in [(b"MAIL", "eq"), (b"AUTOMOBILE", "eq"), (b"AUTOMOBI", "ne"),
+                      (b"M", "lt"), (b"REG AIR", "ge"), (b"\xc3", "gt"), (b"", "le")]:
+        x = M.MapFilter([(addr, "u64"), (ids, "u32")])
+        s, i = x.cols
+        x.where(M.str_compare(s, const, op))
+        x.emit(i, "u32")
+        out, cell = rf.DeviceBuffer(4 * n), rf.u64_cell()
+        x.run(mctx, n, [out], cell)
+        mctx.synchronize()
+        k = rf.read_u64(cell)
+        keep, _ = oracle.mapfilter(x.trees()[0], [], [col, np.arange(n, dtype=np.uint32)], n)
+        assert k == int(keep.sum()), (const, op)
+        assert np.array_equal(out.download(np.uint32, k), np.flatnonzero(keep).astype(np.uint32))
+
+
+def test_plan_string_filter(rf, ctx):
+    """An RA plan filtering a STRING column against string constants
+    (shipmode IN ('MAIL', 'SHIP') as two filters and a union)."""
+    from redfox_amd import raplan
+
+    rng = np.random.default_rng(33)
+    modes = [b"TRUCK", b"MAIL", b"SHIP", b"AIR", b"REG AIR"]
+    rows = [(modes[int(m)], int(v)) for m, v in zip(rng.integers(0, 5, 5000),
+                                                     rng.integers(0, 10 ** 6, 5000))]
+    text = b"".join(b"%s|%d|\n" % r for r in rows)
+    raw = np.frombuffer(text, dtype=np.uint8)
+    buf = rf.DeviceBuffer(len(raw) + 16)
+    buf.upload(raw)
+    (addr, val), n, _ = loaders.parse_tbl(ctx, buf, ["str", "i64"], "|", len(raw))
+    b = raplan.Builder("modes")
+    b.var("L", ["string"], ["int64"]).var("M", fields=["int64"]).var("S", fields=["int64"])
+    b.var("U", fields=["int64"])
+    s = b.sequence(0)
+    s.mapfilter("M", "L", 2, b.comp("eq", b.idx(0, "string"), b.const_string("MAIL")),
+                [b.idx(1, "int64")])
+    s.mapfilter("S", "L", 2, b.comp("eq", b.const_string("SHIP"), b.idx(0, "string")),
+                [b.idx(1, "int64")])
+    s.setop("U", "union", "M", "S")
+    s.halt()
+    p = raplan.Plan(ctx, b.entry_exit(0, 0).bytes())
+    p.set_columns("L", [(addr, "str"), (val, "i64")], n)
+    p.run()
+    assert p.get("U")[0].tolist() == sorted({v for m, v in rows if m in (b"MAIL", b"SHIP")})
