@@ -623,6 +623,11 @@ int rfx_harmony_call(const char* kernel_name, const uint64_t* args,
                               -128..127 -> 1..255, so chunks compare (unsigned)
                               in compare_string2's order.  R[a] must be 0 or the
                               address of a NUL-terminated string in device memory */
+#define RFX_MF_LIKE 64     /* string at R[a] LIKE pattern at R[b] (addresses as for
+                              SCHUNK): '%' matches any run; the pattern's head
+                              before the first '%' and tail after the last must
+                              match the string's ends, its middle words are found
+                              leftmost in turn (Comparisons.h:12-372); 1 or 0 */
 #define RFX_MF_OUT 70      /* output column b at the row's rank = R[a], converted to
                               the column type (integers truncated, F32 rounded) */
 

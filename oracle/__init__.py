@@ -339,6 +339,11 @@ def mapfilter_eval(tree, cols):
                 codes = [(b ^ 0x80) + (1 if b >= 0x80 else 0) for b in s] + [0] * (8 * k + 8)
                 out.append(int.from_bytes(bytes(codes[8 * k:8 * k + 8]), "big"))
             return np.array(out, dtype=np.uint64).view(np.int64), "u"
+        if tag == "like":
+            if tree[1][0] != "col" or cols[tree[1][1]].dtype != object:
+                raise ValueError("like reads a column of byte strings")
+            return np.array([int(like(s or b"", tree[2])) for s in cols[tree[1][1]]],
+                            dtype=np.int64), "i"
         if tag == "int":
             return np.array(tree[1] & 0xFFFFFFFFFFFFFFFF, dtype=np.uint64).view(np.int64), "i"
         if tag == "float":
@@ -555,3 +560,51 @@ def find_bounds_string(left, right):
         cnt.append(bisect.bisect_right(rk, k) - lo)
     lc = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int64) if cnt else np.zeros(0, np.int64)
     return np.array(lb, dtype=np.int64), lc, int(sum(cnt))
+
+
+def like(s: bytes, p: bytes) -> bool:
+    """The reference's LIKE (Comparisons.h:12-372), '%' the only wildcard:
+    the pattern's head must start the string and its tail end it (compared
+    from the ends, not past the head), then each middle word is found
+    leftmost from where the previous one ended."""
+    s = s.split(b"\0", 1)[0]
+    p = p.split(b"\0", 1)[0]
+    s0 = p0 = 0
+    if not p.startswith(b"%"):
+        i = 0
+        while i < len(s) and i < len(p) and p[i:i + 1] != b"%":
+            if s[i] != p[i]:
+                return False
+            i += 1
+        if i < len(p) and p[i:i + 1] != b"%":
+            return False
+        s0 = p0 = i
+    if p0 >= len(p):
+        return s0 >= len(s)
+    p1 = len(p)
+    if not p.endswith(b"%"):
+        i, j = len(s), len(p)
+        while i > s0 and j > p0:
+            i -= 1
+            j -= 1
+            if p[j:j + 1] == b"%":
+                break
+            if s[i] != p[j]:
+                return False
+        if p[j:j + 1] != b"%":
+            return False
+        p1 = j + 1
+    i, j = s0, p0
+    while j < p1:
+        while p[j:j + 1] == b"%":
+            j += 1
+        if j >= p1:
+            break
+        w0 = j
+        while j < p1 and p[j:j + 1] != b"%":
+            j += 1
+        at = s.find(p[w0:j], i)
+        if at < 0:
+            return False
+        i = at + (j - w0)
+    return j == p1

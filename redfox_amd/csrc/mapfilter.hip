@@ -84,7 +84,7 @@ bool valid_code(uint32_t c) {
   return c <= RFX_MF_LDX || (c >= RFX_MF_IADD && c <= RFX_MF_IMOD) ||
          (c >= RFX_MF_FADD && c <= RFX_MF_FDIV) || (c >= RFX_MF_I2F && c <= RFX_MF_F2I) ||
          (c >= RFX_MF_ILT && c <= RFX_MF_FNE) || (c >= RFX_MF_AND && c <= RFX_MF_NOT) ||
-         (c >= RFX_MF_ADD_MONTH && c <= RFX_MF_SCHUNK) || c == RFX_MF_OUT;
+         (c >= RFX_MF_ADD_MONTH && c <= RFX_MF_LIKE) || c == RFX_MF_OUT;
 }
 
 // ---------------------------------------------------------------------------

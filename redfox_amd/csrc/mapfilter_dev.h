@@ -163,6 +163,62 @@ __device__ inline uint64_t mf_schunk(uint64_t p, uint64_t k) {
   return v;
 }
 
+// LIKE with '%' wildcards (Comparisons.h:12-372): the pattern's head (before
+// its first '%') must start the string, its tail (after its last '%') must
+// end it, and each middle word is then found leftmost from where the last
+// one ended (a search that is not kept out of the tail, as there).  Null
+// addresses read as the empty string.
+__device__ inline uint64_t mf_like(uint64_t ps, uint64_t pp) {
+  const uint8_t empty = 0;
+  const uint8_t* s = ps ? reinterpret_cast<const uint8_t*>(ps) : &empty;
+  const uint8_t* p = pp ? reinterpret_cast<const uint8_t*>(pp) : &empty;
+  uint32_t s0 = 0, p0 = 0;
+  if (p[0] != '%') {  // head
+    uint32_t i = 0;
+    while (s[i] && p[i] && p[i] != '%') {
+      if (s[i] != p[i]) return 0;
+      ++i;
+    }
+    if (p[i] && p[i] != '%') return 0;
+    s0 = p0 = i;
+  }
+  if (p[p0] == 0) return s[s0] == 0;
+  uint32_t sl = 0, pl = 0;
+  while (s[sl]) ++sl;
+  while (p[pl]) ++pl;
+  uint32_t p1 = pl;
+  if (p[pl - 1] != '%') {  // tail, compared from the ends
+    uint32_t i = sl, j = pl;
+    while (i > s0 && j > p0) {
+      --i;
+      --j;
+      if (p[j] == '%') break;
+      if (s[i] != p[j]) return 0;
+    }
+    if (p[j] != '%') return 0;
+    p1 = j + 1;
+  }
+  uint32_t i = s0, j = p0;
+  while (j < p1) {  // middle words, each found leftmost from i
+    while (p[j] == '%') ++j;
+    if (j >= p1) break;
+    const uint32_t w0 = j;
+    while (j < p1 && p[j] != '%') ++j;
+    const uint32_t wl = j - w0;
+    bool found = false;
+    for (uint32_t a = i; s[a] && !found; ++a) {
+      uint32_t k = 0;
+      while (k < wl && s[a + k] && s[a + k] == p[w0 + k]) ++k;
+      if (k == wl) {
+        i = a + wl;
+        found = true;
+      }
+    }
+    if (!found) return 0;
+  }
+  return j == p1;
+}
+
 // The value of one op that is neither a column load (LDC) nor a store
 // (OUT), from its operand registers a and b and its immediate.  With a
 // constant code the switch folds to the one operation.
@@ -197,6 +253,7 @@ __device__ __forceinline__ uint64_t mf_apply(uint32_t code, uint64_t a, uint64_t
     case RFX_MF_OR: return static_cast<uint64_t>((a != 0) | (b != 0));
     case RFX_MF_NOT: return a == 0;
     case RFX_MF_SCHUNK: return mf_schunk(a, imm);
+    case RFX_MF_LIKE: return mf_like(a, b);
     case RFX_MF_ADD_MONTH:
     case RFX_MF_ADD_YEAR:
     case RFX_MF_PART_YEAR: {

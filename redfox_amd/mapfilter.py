@@ -27,7 +27,7 @@ TYPES = {"u8": 1, "u16": 2, "u32": 3, "u64": 4, "i8": 5, "i16": 6, "i32": 7, "i6
 OP = dict(NOP=0, LDC=1, LDI=2, MOV=3, LDX=4, IADD=10, ISUB=11, IMUL=12, IDIV=13, IMOD=14, FADD=20,
           FSUB=21, FMUL=22, FDIV=23, I2F=30, U2F=31, F2I=32, ILT=40, ILE=41, IEQ=42, INE=43,
           ULT=44, ULE=45, FLT=46, FLE=47, FEQ=48, FNE=49, AND=50, OR=51, NOT=52, ADD_MONTH=60,
-          ADD_YEAR=61, PART_YEAR=62, SCHUNK=63, OUT=70)
+          ADD_YEAR=61, PART_YEAR=62, SCHUNK=63, LIKE=64, OUT=70)
 MAX_OPS, MAX_COLS, MAX_OUTS, REGS = 96, 16, 8, 16
 
 
@@ -124,6 +124,24 @@ class Const(Expr):
 
     def tree(self):
         return ("float", float(self.v)) if self.type == "f" else ("int", int(self.v))
+
+
+class Like(Expr):
+    """The string at address s LIKE the pattern at address p (LIKE op);
+    `pattern` is the pattern's bytes, for the expression tree."""
+    type = "i"
+
+    def __init__(self, s, p, pattern: bytes):
+        if "f" in (s.type, p.type):
+            raise ValueError("LIKE takes string addresses")
+        self.s, self.p, self.pattern = s, p, bytes(pattern)
+
+    def tree(self):
+        return ("like", self.s.tree(), self.pattern)
+
+
+def like(s, p, pattern: bytes) -> Expr:
+    return Like(lift(s), lift(p), pattern)
 
 
 def lift(v) -> Expr:
@@ -279,6 +297,10 @@ class _Gen:
             if op in ("eq", "ne"):
                 pre = "F" if t == "f" else "I"
             self.emit(pre + op.upper(), r, a, b)
+        elif isinstance(e, Like):
+            self.gen(e.s, r)
+            self.gen(e.p, r + 1)
+            self.emit("LIKE", r, r, r + 1)
         elif isinstance(e, Logic):
             self.gen(e.x, r)
             self.gen(e.y, r + 1)

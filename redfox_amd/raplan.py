@@ -428,10 +428,31 @@ def lower_exp(exp, col) -> Tuple[object, str]:
 _CMP = {0: "eq", 1: "ne", 2: "lt", 3: "le", 4: "gt", 5: "ge"}
 
 
-def lower_compare(cmp, col):
-    """A filter expression, or None for ALWAYS."""
+def test_patterns(cmp) -> List[bytes]:
+    """The LIKE patterns of a predicate's string tests, in order."""
+    if cmp.tag == 1:
+        return [cmp.test.ops[1].constExp.literal.string_constant.value.encode()]
+    if cmp.tag == 2:
+        return test_patterns(cmp.andComp.and1) + test_patterns(cmp.andComp.and2)
+    return []
+
+
+def lower_compare(cmp, col, pattern=None):
+    """A filter expression, or None for ALWAYS.  pattern(bytes) gives the
+    expression of a LIKE pattern's device address."""
     if cmp.tag == 3:
         return None
+    if cmp.tag == 1:  # string test: like / notlike (RelationalAlgebraCompiler.cpp:953-970)
+        t = cmp.test
+        if t.testtype != 0 or t.testname not in ("like", "notlike") or len(t.ops) != 2 \
+                or t.ops[1].tag != 0 or t.ops[1].constExp.literal.kind != 3 or pattern is None:
+            raise ValueError(f"string test {t.testname!r} is not supported")
+        a, ta = lower_exp(t.ops[0], col)
+        if ta != "str":
+            raise ValueError("LIKE reads a string column")
+        pat = t.ops[1].constExp.literal.string_constant.value.encode()
+        e = M.like(a, pattern(pat), pat)
+        return e if t.testname == "like" else ~e
     if cmp.tag == 0:
         op = _CMP[cmp.comp.comparison]
         e1, e2 = cmp.comp.op1, cmp.comp.op2
@@ -454,10 +475,10 @@ def lower_compare(cmp, col):
         return a.eq(b) if op == "eq" else a.ne(b) if op == "ne" else \
             {"lt": a < b, "le": a <= b, "gt": a > b, "ge": a >= b}[op]
     if cmp.tag == 2:
-        x = lower_compare(cmp.andComp.and1, col)
-        y = lower_compare(cmp.andComp.and2, col)
+        x = lower_compare(cmp.andComp.and1, col, pattern)
+        y = lower_compare(cmp.andComp.and2, col, pattern)
         return y if x is None else x if y is None else x & y
-    raise ValueError("string tests are not supported")
+    raise ValueError(f"compare tag {cmp.tag} is not supported")
 
 
 # ---------------------------------------------------------------------------
@@ -684,9 +705,20 @@ class Plan:
             raise ValueError(f"MAPFILTER over {node.srcA}: width {node.srcWidth}, "
                              f"relation has {len(src.cols)} columns")
 
+        # LIKE patterns: NUL-terminated in device memory, each address in a
+        # one-row column after the relation's columns.
+        pats = test_patterns(node.predicate)
+        pat_bufs, pat_cols = [], {}
+        for k, pat in enumerate(pats):
+            text = self._buf(len(pat) + 1)
+            text.upload(np.frombuffer(pat + b"\0", dtype=np.uint8))
+            cell = self._cell(text.ptr)
+            pat_bufs += [text, cell]
+            pat_cols.setdefault(pat, len(src.cols) + k)
+
         def build(x):
             col = lambda i: (x.cols[i], src.types[i])  # noqa: E731
-            f = lower_compare(node.predicate, col)
+            f = lower_compare(node.predicate, col, lambda pat: x.gather(pat_cols[pat], 0))
             if f is not None:
                 x.where(f)
             for e in node.reordering:
@@ -697,8 +729,11 @@ class Plan:
         probe = M.MapFilter([(b, _mft(t)) for b, t in src.cols])
         types = [lower_exp(e, lambda i: (probe.cols[i], src.types[i]))[1] for e in node.reordering]
         if src.n == 0:
+            self._release(*pat_bufs)
             return self._empty(types)
-        out = self._mapfilter(list(src.cols), src.n, build, types)
+        cols = list(src.cols) + [(pat_bufs[2 * k + 1], "u64", 1) for k in range(len(pats))]
+        out = self._mapfilter(cols, src.n, build, types)
+        self._release(*pat_bufs)
         out.sorted_by = _identity_prefix(node.reordering, src.sorted_by)  # a filter keeps order
         return out
 
@@ -1050,6 +1085,15 @@ class Builder:
         c.comp.comparison = {"eq": 0, "ne": 1, "lt": 2, "le": 3, "gt": 4, "ge": 5}[op]
         c.comp.op1.CopyFrom(a)
         c.comp.op2.CopyFrom(b)
+        return c
+
+    def test(self, name: str, a, pattern: str):
+        """A string test (like / notlike) of expression a against a pattern."""
+        c = self.m["Compare"](tag=1)
+        c.test.testtype = 0  # FString
+        c.test.testname = name
+        c.test.ops.add().CopyFrom(a)
+        c.test.ops.add().CopyFrom(self.const_string(pattern))
         return c
 
     def and_(self, x, y):

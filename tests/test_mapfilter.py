@@ -242,3 +242,34 @@ def test_schunk_chunk_index_is_bounded():
     assert L.rfx_mapfilter(ctypes.c_void_p(8), cols, 1, 10, ctypes.byref(prog), None, 0,
                            ctypes.c_void_p(16)) == -1
     assert b"chunk" in L.rfx_last_error()
+
+
+def test_like_against_regex_where_they_agree():
+    """The oracle's LIKE (the reference's '%' algorithm) equals a regular
+    expression match for patterns without a tail (no '%', or ending in
+    '%').  With a tail the reference differs from SQL in two ways, and so
+    does the oracle (known answers): a middle word may be found overlapping
+    the tail, and a tail that consumes the whole string fails (its loop
+    stops before reaching the '%')."""
+    import re
+    rng = np.random.default_rng(3)
+    alpha = b"ab%"
+    for _ in range(3000):
+        s = rng.choice(list(b"abc"), int(rng.integers(0, 7))).astype(np.uint8).tobytes()
+        p = rng.choice(list(alpha), int(rng.integers(0, 7))).astype(np.uint8).tobytes()
+        words = p.split(b"%")
+        middle = [w for w in words[1:-1] if w] if len(words) > 2 else []
+        if words[-1] and len(words) > 1:
+            continue  # a tail: the reference's tail check has its own quirks (below)
+        rx = b".*".join(re.escape(w) for w in words)
+        assert oracle.like(s, p) == bool(re.fullmatch(rx, s, re.S)), (s, p)
+    assert oracle.like(b"abc", b"a%bc%c")          # the middle word overlaps the tail
+    assert not re.fullmatch(b"a.*bc.*c", b"abc")
+    assert oracle.like(b"caba", b"%a%aba") and not re.fullmatch(b".*a.*aba", b"caba")
+    assert not oracle.like(b"ab", b"%ab") and oracle.like(b"xab", b"%ab")
+    assert oracle.like(b"abxcd", b"ab%cd") and not oracle.like(b"abcd", b"ab%cd")
+    assert oracle.like(b"", b"%") and oracle.like(b"", b"") and not oracle.like(b"x", b"")
+    x = M.MapFilter([(_Buf(), "u64"), (_Buf(), "u64", 1)])
+    x.where(M.like(x.cols[0], x.gather(1, 0), b"a%"))
+    p = x.program()
+    assert M.OP["LIKE"] in [p.ops[i].code for i in range(p.n_ops)]

@@ -203,3 +203,69 @@ def test_plan_string_filter(rf, ctx):
     p.set_columns("L", [(addr, "str"), (val, "i64")], n)
     p.run()
     assert p.get("U")[0].tolist() == sorted({v for m, v in rows if m in (b"MAIL", b"SHIP")})
+
+
+def test_mapfilter_like(rf, mctx):
+    """LIKE / NOT LIKE with '%' patterns in both kernels against the oracle
+    (the reference's algorithm), over strings parsed from text."""
+    from redfox_amd import mapfilter as M
+
+    rng = np.random.default_rng(41)
+    words = [b"special", b"requests", b"pending", b"deposits", b"carefully", b"ab", b"a", b""]
+    rows = [b" ".join(words[int(i)] for i in rng.integers(0, len(words), int(rng.integers(0, 5))))
+            for _ in range(20_000)]
+    text = b"".join(b"%d|%s|\n" % (i, s) for i, s in enumerate(rows))
+    raw = np.frombuffer(text, dtype=np.uint8)
+    buf = rf.DeviceBuffer(len(raw) + 16)
+    buf.upload(raw)
+    (ids, addr), n, _ = loaders.parse_tbl(mctx, buf, ["u32", "str"], "|", len(raw))
+    col = np.empty(n, dtype=object)
+    col[:] = rows
+    for pat, neg in [(b"%special%requests%", False), (b"special%", False), (b"%deposits", True),
+                     (b"%a%b", False), (b"a", False), (b"%", True), (b"", False),
+                     (b"carefully%ab%ab", False)]:
+        pbuf = rf.DeviceBuffer(len(pat) + 16)
+        pbuf.upload(np.frombuffer(pat + b"\0", dtype=np.uint8))
+        pcol = rf.DeviceBuffer.from_numpy(np.array([pbuf.ptr], dtype=np.uint64))
+        x = M.MapFilter([(addr, "u64"), (ids, "u32"), (pcol, "u64", 1)])
+        s, i, _ = x.cols
+        e = M.like(s, x.gather(2, 0), pat)
+        x.where(~e if neg else e)
+        x.emit(i, "u32")
+        out, cell = rf.DeviceBuffer(4 * n), rf.u64_cell()
+        x.run(mctx, n, [out], cell)
+        mctx.synchronize()
+        k = rf.read_u64(cell)
+        keep, _ = oracle.mapfilter(x.trees()[0], [], [col], n)
+        assert k == int(keep.sum()), (pat, neg)
+        assert np.array_equal(out.download(np.uint32, k), np.flatnonzero(keep).astype(np.uint32))
+
+
+def test_plan_like_tests(rf, ctx):
+    from redfox_amd import raplan
+
+    rng = np.random.default_rng(43)
+    comments = [b"quickly special requests", b"final deposits", b"special packages",
+                b"even requests sleep", b"blithely special accounts requests"]
+    rows = [(comments[int(c)], int(v)) for c, v in zip(rng.integers(0, 5, 3000),
+                                                        rng.integers(0, 10 ** 6, 3000))]
+    text = b"".join(b"%s|%d|\n" % r for r in rows)
+    raw = np.frombuffer(text, dtype=np.uint8)
+    buf = rf.DeviceBuffer(len(raw) + 16)
+    buf.upload(raw)
+    (addr, val), n, _ = loaders.parse_tbl(ctx, buf, ["str", "i64"], "|", len(raw))
+    b = raplan.Builder("like")
+    b.var("C", ["string"], ["int64"]).var("L", fields=["int64"]).var("N", fields=["int64"])
+    s = b.sequence(0)
+    s.mapfilter("L", "C", 2, b.test("like", b.idx(0, "string"), "%special%requests%"),
+                [b.idx(1, "int64")])
+    s.mapfilter("N", "C", 2, b.and_(b.test("notlike", b.idx(0, "string"), "%requests%"),
+                                    b.comp("lt", b.idx(1, "int64"), b.const_int(500_000))),
+                [b.idx(1, "int64")])
+    s.halt()
+    p = raplan.Plan(ctx, b.entry_exit(0, 0).bytes())
+    p.set_columns("C", [(addr, "str"), (val, "i64")], n)
+    p.run()
+    assert p.get("L")[0].tolist() == [v for c, v in rows if oracle.like(c, b"%special%requests%")]
+    assert p.get("N")[0].tolist() == [v for c, v in rows
+                                      if not oracle.like(c, b"%requests%") and v < 500_000]
