@@ -211,6 +211,13 @@ int rfx_sort_strings(rfx_ctx* ctx, uint64_t* addresses, uint64_t* scratch, uint6
  * relations whose other columns follow the key. */
 int rfx_sort_strings_rows(rfx_ctx* ctx, const uint64_t* addresses, uint32_t* rows,
                           uint64_t* scratch, uint64_t n);
+/* Number the runs of equal strings in a sorted array of string addresses:
+ * group[i] = number of distinct strings before row i's string (0, 0, 1,
+ * ...), *n_groups (device u64) the count; scratch holds n int32.  No
+ * reference counterpart: it turns strings into sorted integer group keys
+ * for GROUP BY on a string column. */
+int rfx_string_groups(rfx_ctx* ctx, const uint64_t* sorted, uint64_t n, uint32_t* group,
+                      int32_t* scratch, uint64_t* n_groups);
 /* Load-balanced expansion (redfox::join, ModernGPUJoin.cu:665-733):
  * for g < total, left_idx[g] = i with left_count[i] <= g <
  * left_count[i] + count[i], right_idx[g] = lower_bound[i] + g -

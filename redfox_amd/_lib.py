@@ -114,6 +114,7 @@ SIGNATURES = {
     "rfx_join_find_bounds_string": (I, [P, P, P, P, P, U64, P, U64]),
     "rfx_sort_strings": (I, [P, P, P, U64]),
     "rfx_sort_strings_rows": (I, [P, P, P, P, U64]),
+    "rfx_string_groups": (I, [P, P, U64, P, P, P]),
     "rfx_hir_load": (I, [POINTER(c_void_p), P, U64]),
     "rfx_hir_destroy": (None, [P]),
     "rfx_hir_info": (I, [P, P]),

@@ -174,10 +174,57 @@ __global__ __launch_bounds__(kSNT) void k_str_chunk(const uint64_t* __restrict__
   key[i] = k;
 }
 
+// Run heads of sorted strings: 1 where a string differs from the one
+// before it (sorted, so where the one before is less), 0 elsewhere.
+__global__ __launch_bounds__(kSNT) void k_str_heads(const uint64_t* __restrict__ addr, uint64_t n,
+                                                    int32_t* __restrict__ head) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kSNT + threadIdx.x;
+  if (i >= n) return;
+  head[i] = i == 0 || str_less(addr[i - 1], addr[i]) ? 1 : 0;
+}
+
+// After the exclusive scan of the heads: group of row i = heads up to and
+// including i, less one (the scan of row i + 1, or the total for the last).
+__global__ __launch_bounds__(kSNT) void k_str_groups(const int32_t* __restrict__ excl, uint64_t n,
+                                                     const uint64_t* __restrict__ total,
+                                                     uint32_t* __restrict__ group) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kSNT + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t incl = i + 1 < n ? static_cast<uint32_t>(excl[i + 1]) : *total;
+  group[i] = static_cast<uint32_t>(incl - 1);
+}
+
 }  // namespace
 }  // namespace rfx
 
 using namespace rfx;
+
+extern "C" int rfx_string_groups(rfx_ctx* ctx, const uint64_t* sorted, uint64_t n, uint32_t* group,
+                                 int32_t* scratch, uint64_t* n_groups) {
+  if (!ctx || !n_groups || (n && (!sorted || !group || !scratch)))
+    return fail(RFX_EINVAL, "rfx_string_groups: null argument");
+  if (n >= (1ull << 31)) return fail(RFX_ERANGE, "rfx_string_groups: more than 2^31 - 1 rows");
+  if (n == 0) return fill_u32(ctx, reinterpret_cast<uint32_t*>(n_groups), 2, 0);
+  const uint32_t grid = static_cast<uint32_t>((n + kSNT - 1) / kSNT);
+  k_str_heads<<<grid, kSNT, 0, ctx->stream>>>(sorted, n, scratch);
+  RFX_LAUNCHED();
+  const uint64_t ntiles = (n + kCTile - 1) / kCTile;
+  Carve c(nullptr);
+  c.take<uint64_t>(ntiles);
+  c.take<uint32_t>(1);
+  RFX_TRY(ensure_workspace(ctx, c.off));
+  Carve w(ctx->ws);
+  uint64_t* status = w.take<uint64_t>(ntiles);
+  uint32_t* ticket = w.take<uint32_t>(1);
+  RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(status),
+                   (reinterpret_cast<char*>(ticket + 1) - reinterpret_cast<char*>(status)) / 4, 0));
+  k_count_scan<<<static_cast<uint32_t>(ntiles), kSNT, 0, ctx->stream>>>(
+      scratch, n, static_cast<uint32_t>(ntiles), n_groups, status, ticket);
+  RFX_LAUNCHED();
+  k_str_groups<<<grid, kSNT, 0, ctx->stream>>>(scratch, n, n_groups, group);
+  RFX_LAUNCHED();
+  return RFX_OK;
+}
 
 extern "C" int rfx_sort_strings(rfx_ctx* ctx, uint64_t* addresses, uint64_t* scratch, uint64_t n) {
   if (!ctx || (n && (!addresses || !scratch)))

@@ -834,6 +834,10 @@ class Plan:
         dom_types = [src.types[i] for i in dom]
         if src.n == 0:
             return self._empty(dom_types + [out_t])
+        if "str" in dom_types:
+            if len(dom) != 1:
+                raise ValueError("an AGG over a string domain takes that one domain column")
+            return self._agg_string(src, dom[0], which, vcol, out_t)
         kt, bits, fields = key_layout(dom_types)
         kbytes = 4 if kt == "u32" else 8
         n = src.n
@@ -871,6 +875,50 @@ class Plan:
                               dom_types + [out_t])
         self._release(keys, vals, ncell, nout, okeys, ovals)
         out.sorted_by = len(dom)
+        return out
+
+    def _agg_string(self, src: Relation, d: int, which: str, vcol, out_t: str) -> Relation:
+        """AGG grouped by one string column: rows in string order, runs of
+        equal strings numbered (rfx_string_groups), each run reduced; a
+        group's string is its smallest address (addresses are below 2^53,
+        exact as doubles)."""
+        n = src.n
+        perm, scratch = self._alloc("u32", n), self._alloc("u64", n + 1)
+        self.ctx.iota_u32(perm, n)
+        self.ctx.sort_strings_rows(src.cols[d][0], perm, scratch, n)
+        cols = [(perm, "u32"), (src.cols[d][0], "u64", n)]
+        if vcol is not None:
+            cols.append((src.cols[vcol][0], src.cols[vcol][1], n))
+
+        def gather(x):
+            row = x.cols[0]
+            a = x.gather(1, row)
+            x.emit(a, "u64")
+            x.emit(M.to_float(a), "f64")
+            if vcol is not None:
+                x.emit(M.to_float(x.gather(2, row)), "f64")
+        g = self._mapfilter(cols, n, gather, ["u64", "f64"] + (["f64"] if vcol is not None else []))
+        addr, addr_f = g.cols[0][0], g.cols[1][0]
+        vals = g.cols[2][0] if vcol is not None else None
+        group, heads, ngroups = self._alloc("u32", n), self._alloc("i32", n), self._buf(8)
+        self.ctx.string_groups(addr, n, group, heads, ngroups)
+        ncell, nout, nout2 = self._cell(n), self._buf(8), self._buf(8)
+        okeys, okeys2, oaddr = self._alloc("u32", n), self._alloc("u32", n), self._alloc("f64", n)
+        if which == "count":
+            ovals, vt = self._alloc("u32", n), "u32"
+            self.ctx.count_by_key(okeys, ovals, nout, group, ncell, n, 4)
+        else:
+            ovals, vt = self._alloc("f64", n), "f64"
+            self.ctx.reduce_by_key_f64(okeys, ovals, nout, group, vals, ncell, n, 4,
+                                       {"total": 0, "min": 1, "max": 2}[which])
+        self.ctx.reduce_by_key_f64(okeys2, oaddr, nout2, group, addr_f, ncell, n, 4, 1)
+        groups = rfx.read_u64(nout)
+        out = self._mapfilter([(oaddr, "f64"), (ovals, vt)], groups, lambda x: (
+            x.emit(M.to_int(x.cols[0]), "u64"), x.emit(_as(x.cols[1], out_t), out_t)),
+            ["str", out_t])
+        self._release(perm, scratch, addr, addr_f, vals, group, heads, ngroups, ncell, nout, nout2,
+                      okeys, okeys2, oaddr, ovals)
+        out.sorted_by = 0  # string order, not the addresses' order
         return out
 
     def set_operation(self, op: int, a_name: str, b_name: str) -> Relation:

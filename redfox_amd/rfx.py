@@ -224,6 +224,12 @@ class Context:
         _check("rfx_sort_strings_rows", lib().rfx_sort_strings_rows(
             self.handle, addresses.ptr, rows.ptr, scratch.ptr, n))
 
+    def string_groups(self, sorted_addresses, n, group, scratch, n_groups) -> None:
+        """Group numbers (u32) of the runs of equal strings in a sorted
+        address array; scratch holds n int32, n_groups is a device u64."""
+        _check("rfx_string_groups", lib().rfx_string_groups(
+            self.handle, sorted_addresses.ptr, n, group.ptr, scratch.ptr, n_groups.ptr))
+
     def join_main(self, left_idx, right_idx, total, total_cap, lower_bound, left_count,
                   nl) -> None:
         _check("rfx_join_main", lib().rfx_join_main(

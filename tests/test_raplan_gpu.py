@@ -162,3 +162,44 @@ def test_string_key_join_plan(rf, ctx):
     want = sorted((va, vb) for ka, va in a_rows for kb, vb in b_rows if ka == kb)
     assert sorted(zip(x.tolist(), y.tolist())) == want
     assert p.rows("J") == len(want)
+
+
+def test_string_group_by_plan(rf, ctx):
+    """AGG grouped by a STRING column (COUNT, TOTAL, MIN): groups are the
+    distinct strings, each once, with its aggregate."""
+    from redfox_amd import loaders
+
+    rng = np.random.default_rng(51)
+    modes = [b"TRUCK", b"MAIL", b"SHIP", b"AIR", b"REG AIR", b"\xc3\xa9t\xc3\xa9", b"", b"MAILBOX"]
+    rows = [(modes[int(m)], float(v) / 4) for m, v in zip(rng.integers(0, len(modes), 20_000),
+                                                         rng.integers(0, 4000, 20_000))]
+    text = b"".join(b"%s|%r|\n" % (m, v) for m, v in rows)
+    raw = np.frombuffer(text, dtype=np.uint8)
+    buf = rf.DeviceBuffer(len(raw) + 16)
+    buf.upload(raw)
+    (addr, val), n, _ = loaders.parse_tbl(ctx, buf, ["str", "f64"], "|", len(raw))
+    consumed = buf.download(np.uint8, len(raw)).tobytes()
+
+    def string_at(a):
+        off = int(a) - buf.ptr
+        return consumed[off:consumed.index(b"\0", off)]
+    b = raplan.Builder("groupby")
+    b.var("L", ["string"], ["float64"])
+    b.var("C", ["string"], ["int64"]).var("T", ["string"], ["float64"])
+    b.var("M", ["string"], ["float64"])
+    s = b.sequence(0)
+    s.agg("C", "L", 2, [0], "count").agg("T", "L", 2, [0], "total", 1, 1)
+    s.agg("M", "L", 2, [0], "min", 1, 1)
+    s.halt()
+    p = raplan.Plan(ctx, b.entry_exit(0, 0).bytes())
+    p.set_columns("L", [(addr, "str"), (val, "f64")], n)
+    p.run()
+    groups = {}
+    for m, v in rows:
+        groups.setdefault(m, []).append(v)
+    for name, f in (("C", len), ("T", sum), ("M", min)):
+        a, v = p.get(name)
+        got = {string_at(x): y for x, y in zip(a, v)}
+        assert len(got) == len(a) == len(groups), name
+        for m, vs in groups.items():
+            assert abs(got[m] - f(vs)) <= 1e-12 * max(1.0, abs(f(vs))), (name, m)
