@@ -725,7 +725,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not supported "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+              "(the kernels run on one GPU's shard of a relation)", file=sys.stderr)
         sys.exit(2)
     import redfox_amd as rf
 
