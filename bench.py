@@ -724,7 +724,7 @@ def main():
                     "besides the headline sort (e.g. mapfilter_2^27)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not supported "
+        print("bench.py: measures one GPU; multi-GPU runs are not supported "
               "(the kernels run on one GPU's shard of a relation)", file=sys.stderr)
         sys.exit(2)
     import redfox_amd as rf
