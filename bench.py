@@ -549,10 +549,10 @@ def bench_batched(rf, ctx, reps=5):
     return res
 
 
-def bench_served(rf, ctx, n_req=200, slots=8, rates=(0.25, 0.5, 0.75, 0.9)):
+def bench_served(rf, ctx, n_req=200, slots=8, rates=(0.25, 0.5, 0.75, 0.9), exp=(5.0, 8.0)):
     """Mixed requests (30% select-project, 30% join-aggregate, 20% three-way
     join chain, 20% u64 set operations) over device-resident relations of
-    floor(10^U[5,8]) rows: saturation throughput of the continuous server
+    floor(10^U[exp]) rows: saturation throughput of the continuous server
     (all requests arriving at once) against the plain one-request-at-a-time
     form with host synchronisation after every operator, then Poisson
     arrivals at fractions of the saturation rate for p50/p99 latency."""
@@ -560,7 +560,7 @@ def bench_served(rf, ctx, n_req=200, slots=8, rates=(0.25, 0.5, 0.75, 0.9)):
 
     free, _ = mem_info(rf)
     srv0 = S.Server(0, slots=slots, budget_bytes=1 << 30)
-    cat = S.Catalog(ctx, srv0, per_kind=4, exp=(5.0, 8.0))
+    cat = S.Catalog(ctx, srv0, per_kind=4, exp=exp)
     free, _ = mem_info(rf)
     budget = int(min(200 << 30, free - (16 << 30)))
     srv = S.Server(0, slots=slots, budget_bytes=budget)
@@ -754,6 +754,8 @@ def main():
         ("setop_2^27", "set operations 2^27 + 2^27 u64", lambda: bench_setop(rf, ctx)),
         ("batched_512", "batched 512 join-aggregate instances", lambda: bench_batched(rf, ctx)),
         ("served_mix", "served mixed requests", lambda: bench_served(rf, ctx)),
+        ("served_small_mix", "served mixed small requests",
+         lambda: bench_served(rf, ctx, n_req=2000, exp=(3.0, 5.0))),
     ]
     only = set(args.only.split(",")) if args.only else None
     if only and not only <= {name for name, _, _ in lines}:
