@@ -143,8 +143,9 @@ def bench_sort_u64_zipf(rf, ctx, iters=3):
     v = rf.DeviceBuffer(4 * n)
     L = rf.lib()
     times = []
-    ctx.profile(True)
-    for _ in range(iters + 1):
+    for i in range(iters + 1):
+        if i == 1:
+            ctx.profile(True)  # per-launch figures over the timed calls only
         L.rfx_memcpy_dtod(k.ptr, k0.ptr, 8 * n)
         ctx.iota_u32(v, n)
         ctx.synchronize()
@@ -295,7 +296,7 @@ def bench_mapfilter(rf, ctx, iters=5):
 
 
 PROF_TAGS = ("join.find_bounds", "join.expand", "join.expand_gather", "join.expand_project",
-             "join.gather", "mapfilter", "q1.partial", "rbk.down", "rbk.up", "reduce.single",
+             "join.gather", "mapfilter", "q1.partial", "rbk.pass", "reduce.single",
              "select", "setop", "sort.hist", "sort.onesweep", "unique")
 
 
@@ -421,27 +422,27 @@ def bench_agg(rf, ctx, iters=5):
     ctx.sort_pairs(keys, vals, n, 4, 20, 8)
     ok, ov, oc = rf.DeviceBuffer(4 * groups), rf.DeviceBuffer(8 * groups), rf.DeviceBuffer(4 * groups)
     nout, ncell = rf.u64_cell(), rf.u64_cell(n)
+    # One untimed call first (code-object load, workspace), so that the
+    # timed calls and the per-launch figures cover the same calls.
+    ctx.reduce_count_by_key_f64(ok, ov, oc, nout, keys, vals, ncell, n, 4, 0, out_cap=groups)
     ctx.synchronize()
     ctx.profile(True)
     times = []
-    for _ in range(iters + 1):
+    for _ in range(iters):
         t = rf.Timer(ctx)
         t.begin()
         ctx.reduce_count_by_key_f64(ok, ov, oc, nout, keys, vals, ncell, n, 4, 0, out_cap=groups)
         times.append(t.end())
         t.close()
-    # Two passes (upsweep, downsweep) per call; the roofline charges the
-    # algorithmic bytes (each row read once, each group written once) to
-    # their combined time.
-    lu, mu = ctx.profile_read("rbk.up")
-    ld, md = ctx.profile_read("rbk.down")
+    # The roofline charges the algorithmic bytes (each row read once, each
+    # group written once) to the reduce kernel's launches.
+    lo, mo = ctx.profile_read("rbk.pass")
     ctx.profile(False)
     g = rf.read_u64(nout)
     cnt = oc.download(np.uint32, g)
-    res = {"n": n, "groups": g, "ms": median(times[1:]), "rows_per_s": n / (median(times[1:]) * 1e-3),
+    res = {"n": n, "groups": g, "ms": median(times), "rows_per_s": n / (median(times) * 1e-3),
            "counts_sum_to_n": bool(int(cnt.sum(dtype=np.uint64)) == n),
-           "roofline": roofline(12 * n + 16 * g, lu, mu + md),
-           "upsweep_ms": mu / max(lu, 1), "downsweep_ms": md / max(ld, 1)}
+           "roofline": roofline(12 * n + 16 * g, lo, mo)}
     for b in (keys, vals, ident, row_off, dim_off, ok, ov, oc, nout, ncell):
         b.close()
     return res
@@ -459,9 +460,10 @@ def bench_select(rf, ctx, iters=5):
     ctx.synchronize()
     hi.close()
     lo.close()
-    ctx.profile(True)
     times = []
-    for _ in range(iters + 1):
+    for i in range(iters + 1):
+        if i == 1:
+            ctx.profile(True)  # per-launch figures over the timed calls only
         t = rf.Timer(ctx)
         t.begin()
         ctx.select(out, nout, tup, n, 16, 64, 32, 2, 1 << 31)
@@ -489,9 +491,10 @@ def bench_setop(rf, ctx, iters=5):
     nout = rf.u64_cell()
     res = {"na": na, "nb": nb}
     for op, name in ((0, "union"), (2, "intersection"), (1, "difference")):
-        ctx.profile(True)
         times = []
-        for _ in range(iters + 1):
+        for i in range(iters + 1):
+            if i == 1:
+                ctx.profile(True)  # per-launch figures over the timed calls only
             t = rf.Timer(ctx)
             t.begin()
             ctx.set_op(op, out, nout, a, na, b, nb, 8)

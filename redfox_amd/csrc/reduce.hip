@@ -5,25 +5,30 @@
 // (Reduce.cu:175-1532: thrust::reduce_by_key with D2H/H2D size copies) and
 // total_double / count / max_double (Reduce.cu:976-1136: thrust::reduce).
 //
-// Reduce-by-key makes three launches that never wait on each other's
-// progress.  Tiles of TILE consecutive rows (boundaries at multiples of
-// TILE from row 0), one per block: a thread folds its VT consecutive rows,
-// a fixed shuffle/LDS tree scans the threads with the segmented operator
-// (h1, s1) . (h2, s2) = (h1 + h2, h2 ? s2 : s1 op s2), and the upsweep
-// writes the tile's aggregate (run heads, open-run value, last head).  One
-// block scans the tile aggregates, each thread folding a fixed chunk of
-// tiles and a fixed tree combining the chunks; the downsweep reads each
-// tile again with its carry-in and writes every run that ends in it, with
-// its key, value and row count (end - head + 1).  Every addition's order
-// is a function of the row count and the data alone, never of scheduling.
+// Reduce-by-key reads the rows once.  Tiles of TILE consecutive rows
+// (boundaries at multiples of TILE from row 0), one per block, numbered by
+// an atomic ticket: a thread folds its VT consecutive rows, a fixed
+// shuffle/LDS tree scans the threads with the segmented operator
+// (h1, s1) . (h2, s2) = (h1 + h2, h2 ? s2 : s1 op s2), a decoupled
+// look-back over head counts (integers, exact in any order) gives the
+// tile's first run index, and the tile writes every run that ends in it,
+// with its key, value and row count (end - head + 1).  The run open at the
+// tile start is written with the tile's part only.  The tile aggregates
+// (run heads, open-run value, last head) are then scanned in two levels by
+// fixed trees: groups of kSpNT consecutive tiles, one block each, and one
+// block over the group totals, each thread folding a fixed chunk of groups.
+// A patch kernel completes each tile's open run with its carry-in.  Every
+// FP64 operation's order is a function of the row count and the data
+// alone, never of scheduling.
 #include "rfx_device.h"
 #include "rfx_internal.h"
 
 namespace rfx {
 namespace {
 
-constexpr int kBNT = 256;            // threads per reduce-by-key tile
+constexpr int kBNT = 512;            // threads per reduce-by-key tile
 constexpr uint32_t kRChunks = 1024;  // single reductions: fixed chunk count
+constexpr uint32_t kSpNT = 1024;     // tile aggregates per spine group (one block)
 
 struct OpSum {
   template <class V> __device__ static V apply(V a, V b) { return a + b; }
@@ -194,30 +199,6 @@ __device__ __forceinline__ Seg<V> tile_scan(const Seg<V>& mine, Seg<V>* s_wave, 
   return total;
 }
 
-template <class K, class V, class Op, int MODE, bool VEC>
-__global__ __launch_bounds__(kBNT) void k_rbk_up(const K* __restrict__ keys,
-                                                 const V* __restrict__ vals, Sizes z,
-                                                 Seg<V>* __restrict__ agg) {
-  constexpr int VT = RbkTune<K>::VT;
-  constexpr int TILE = RbkTune<K>::TILE;
-  __shared__ K s_first[kBNT];
-  __shared__ K s_last[kBNT];
-  __shared__ Seg<V> s_wave[kBNT / kWave];
-  const uint64_t n = rows_of(z);
-  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * TILE;
-  if (base >= n) return;
-  TileRows<K, V, Op, (MODE & 1) != 0, VEC, VT> t;
-  t.load(keys, vals, n, base, base + static_cast<uint64_t>(threadIdx.x) * VT, s_first, s_last);
-  Seg<V> before;
-  bool hb;
-  const Seg<V> total = tile_scan<V, Op>(t.mine, s_wave, &before, &hb);
-  if (threadIdx.x == 0) agg[blockIdx.x] = total;
-}
-
-// One block of kRChunks threads: exclusive segmented scan of the tile
-// aggregates in place (thread t folds tiles [t*C, t*C + C) in order, a
-// fixed Kogge-Stone tree combines the threads; C depends on n only), and
-// the run count.
 template <class V, class Op>
 struct SegE {
   Seg<V> s;
@@ -229,46 +210,103 @@ __device__ __forceinline__ SegE<V, Op> sege_combine(const SegE<V, Op>& a, const 
   if (b.e) return a;
   return {seg_combine<Op>(a.s, b.s), false};
 }
-
 template <class V, class Op>
-__global__ __launch_bounds__(kRChunks) void k_rbk_spine(Seg<V>* __restrict__ agg, Sizes z,
-                                                        uint32_t tile_rows,
-                                                        uint64_t* __restrict__ runs) {
-  __shared__ SegE<V, Op> sm[kRChunks];
-  const uint32_t tid = threadIdx.x;
+__device__ __forceinline__ SegE<V, Op> sege_shfl_up(const SegE<V, Op>& x, uint32_t d) {
+  SegE<V, Op> y;
+  y.s = seg_shfl_up(x.s, d);
+  y.e = __shfl_up(static_cast<int>(x.e), d, kWave) != 0;
+  return y;
+}
+
+// Exclusive scan of one value per thread over a block of NT threads by a
+// fixed tree: shuffles inside each wave, then the wave totals in order.
+// Returns the combination of the threads before this one (empty for
+// thread 0) and sets *total to the block's.
+template <int NT, class V, class Op>
+__device__ __forceinline__ SegE<V, Op> block_scan_excl(SegE<V, Op> incl, SegE<V, Op>* s_wave,
+                                                       SegE<V, Op>* total) {
+  constexpr int NW = NT / kWave;
+  const uint32_t lane = lane_id(), wv = threadIdx.x / kWave;
+#pragma unroll
+  for (uint32_t d = 1; d < kWave; d <<= 1) {
+    const SegE<V, Op> y = sege_shfl_up(incl, d);
+    if (lane >= d) incl = sege_combine(y, incl);
+  }
+  if (lane == kWave - 1) s_wave[wv] = incl;
+  const SegE<V, Op> excl_w = sege_shfl_up(incl, 1);
+  __syncthreads();
+  SegE<V, Op> before{{0, V{}, 0}, true};
+  for (uint32_t i = 0; i < wv; ++i) before = sege_combine(before, s_wave[i]);
+  if (lane) before = sege_combine(before, excl_w);
+  SegE<V, Op> t = s_wave[0];
+  for (int i = 1; i < NW; ++i) t = sege_combine(t, s_wave[i]);
+  *total = t;
+  return before;
+}
+
+// Group pass: block b takes tiles [b*kSpNT, b*kSpNT + kSpNT), writes to
+// ex[t] the combination of the tiles before t in the group (not written for
+// the group's first tile, whose carry is the group's alone; ex may be agg),
+// and writes the group total to grp[b].
+template <class V, class Op>
+__global__ __launch_bounds__(kSpNT) void k_rbk_spine_group(const Seg<V>* agg, Seg<V>* ex,
+                                                           Sizes z, uint32_t tile_rows,
+                                                           Seg<V>* __restrict__ grp) {
+  __shared__ SegE<V, Op> s_wave[kSpNT / kWave];
   const uint64_t n = rows_of(z);
   const uint64_t T = (n + tile_rows - 1) / tile_rows;
+  const uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * kSpNT;
+  if (b0 >= T) return;  // uniform over the block
+  const uint64_t t = b0 + threadIdx.x;
+  SegE<V, Op> mine{{0, V{}, 0}, true};
+  if (t < T) mine = {agg[t], false};
+  SegE<V, Op> total;
+  const SegE<V, Op> before = block_scan_excl<kSpNT, V, Op>(mine, s_wave, &total);
+  if (t < T && !before.e) ex[t] = before.s;
+  if (threadIdx.x == 0) grp[blockIdx.x] = total.s;
+}
+
+// One block of kRChunks threads: exclusive segmented scan of the group
+// totals in place (thread t folds groups [t*C, t*C + C) in order, the block
+// scan combines the threads; C depends on n only), and the run count.
+template <class V, class Op>
+__global__ __launch_bounds__(kRChunks) void k_rbk_spine(Seg<V>* __restrict__ grp, Sizes z,
+                                                        uint32_t tile_rows,
+                                                        uint64_t* __restrict__ runs) {
+  __shared__ SegE<V, Op> s_wave[kRChunks / kWave];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t n = rows_of(z);
+  const uint64_t T = ((n + tile_rows - 1) / tile_rows + kSpNT - 1) / kSpNT;
   const uint64_t C = (T + kRChunks - 1) / kRChunks;
   const uint64_t t0 = tid * C, t1 = t0 + C < T ? t0 + C : T;
   SegE<V, Op> acc{{0, V{}, 0}, true};
-  for (uint64_t t = t0; t < t1; ++t) acc = sege_combine(acc, SegE<V, Op>{agg[t], false});
-  sm[tid] = acc;
-  __syncthreads();
-  for (uint32_t d = 1; d < kRChunks; d <<= 1) {
-    SegE<V, Op> y = acc;
-    if (tid >= d) y = sege_combine(sm[tid - d], acc);
-    __syncthreads();
-    acc = y;
-    sm[tid] = acc;
-    __syncthreads();
-  }
-  SegE<V, Op> carry = tid ? sm[tid - 1] : SegE<V, Op>{{0, V{}, 0}, true};
+  for (uint64_t t = t0; t < t1; ++t) acc = sege_combine(acc, SegE<V, Op>{grp[t], false});
+  SegE<V, Op> total;
+  SegE<V, Op> carry = block_scan_excl<kRChunks, V, Op>(acc, s_wave, &total);
   for (uint64_t t = t0; t < t1; ++t) {
-    const Seg<V> a = agg[t];
-    agg[t] = carry.e ? Seg<V>{0, V{}, 0} : carry.s;
+    const Seg<V> a = grp[t];
+    grp[t] = carry.e ? Seg<V>{0, V{}, 0} : carry.s;
     carry = sege_combine(carry, SegE<V, Op>{a, false});
   }
-  if (tid == kRChunks - 1) *runs = acc.e ? 0 : acc.s.h;
+  if (tid == 0) *runs = total.e ? 0 : total.s.h;
 }
 
+// ---- one pass over the rows, then a patch of one run per tile -------------
+// The rows are read once.  Tiles take numbers from a ticket; a tile's run
+// indices come from a decoupled look-back over head counts (integer sums,
+// exact in any order).  Every run that ends in the tile is written; the run
+// open at the tile start (if it ends here) is written with the tile's own
+// part only: its value from the tile's first row and its row count from
+// the tile start.  The tile aggregates go to `agg`; the two small spine
+// kernels above scan them by a fixed tree, and k_rbk_patch then combines
+// the carry-in value with that run's tile part and adds the rows before
+// the tile.  So no tile waits for another tile's values, and every FP64
+// value is a fixed function of the data.
 template <class K, class V, class Op, int MODE, bool VEC>
-__global__ __launch_bounds__(kBNT) void k_rbk_down(const K* __restrict__ keys,
-                                                   const V* __restrict__ vals, Sizes z,
-                                                   const Seg<V>* __restrict__ carry,
-                                                   K* __restrict__ out_keys,
-                                                   V* __restrict__ out_vals,
-                                                   uint32_t* __restrict__ out_counts,
-                                                   uint64_t out_cap) {
+__global__ __launch_bounds__(kBNT) void k_rbk_onepass(
+    const K* __restrict__ keys, const V* __restrict__ vals, Sizes z, uint64_t* st,
+    Seg<V>* __restrict__ agg, K* __restrict__ out_keys, V* __restrict__ out_vals,
+    uint32_t* __restrict__ out_counts, uint64_t out_cap) {
   constexpr int VT = RbkTune<K>::VT;
   constexpr int TILE = RbkTune<K>::TILE;
   constexpr bool HAS_V = MODE & 1;
@@ -276,34 +314,44 @@ __global__ __launch_bounds__(kBNT) void k_rbk_down(const K* __restrict__ keys,
   __shared__ K s_first[kBNT];
   __shared__ K s_last[kBNT];
   __shared__ Seg<V> s_wave[kBNT / kWave];
+  __shared__ uint64_t s_hx;
+  __shared__ uint32_t s_tile;
+  uint64_t* hst = st + 1;  // st[0]: ticket (low half); hst[t]: head-count word of tile t
+  if (threadIdx.x == 0) s_tile = atomicAdd(reinterpret_cast<uint32_t*>(st), 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
   const uint64_t n = rows_of(z);
-  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * TILE;
-  if (base >= n) return;
+  const uint64_t base = static_cast<uint64_t>(tile) * TILE;
+  if (base >= n) return;  // uniform over the block
   const uint64_t first = base + static_cast<uint64_t>(threadIdx.x) * VT;
   TileRows<K, V, Op, HAS_V, VEC, VT> t;
   t.load(keys, vals, n, base, first, s_first, s_last);
   Seg<V> before;
   bool hb;
-  (void)tile_scan<V, Op>(t.mine, s_wave, &before, &hb);
-  if (!t.any) return;
-  // State entering the thread: the tile's carry-in, then the threads
-  // before it in the tile.
-  const Seg<V> cin = carry[blockIdx.x];  // h = heads before the tile
-  const bool tile_open = base != 0;      // a run is open at every tile start but the first
-  Seg<V> in;
-  bool open;
-  if (hb) {
-    in = tile_open ? seg_combine<Op>(cin, before) : before;
-    open = true;
-  } else {
-    in = cin;
-    open = tile_open;
+  const Seg<V> total = tile_scan<V, Op>(t.mine, s_wave, &before, &hb);
+  if (threadIdx.x < kWave) {
+    if (threadIdx.x == 0) {
+      agg[tile] = total;
+      status_store64(hst + tile, (tile ? kFlagAgg64 : kFlagIncl64) | total.h);
+    }
+    if (tile) {
+      const uint64_t hx = wave_lookback64(hst, tile);
+      if (threadIdx.x == 0) {
+        status_store64(hst + tile, kFlagIncl64 | (hx + total.h));
+        s_hx = hx;
+      }
+    } else if (threadIdx.x == 0) {
+      s_hx = 0;
+    }
   }
-  // Index of the run open on entry, in 64 bits: with no run open it is
-  // -1 and the first head makes it 0.
-  uint64_t run = static_cast<uint64_t>(in.h) - 1;
-  V val = in.s;
-  uint64_t hrow = in.lh - 1;
+  __syncthreads();
+  if (!t.any) return;
+  // Entering the thread: the runs before it (heads before the tile, then
+  // in the tile); the open run's value and head row within the tile only.
+  uint64_t run = s_hx + (hb ? before.h : 0u) - 1;
+  V val = hb ? before.s : V{};
+  bool open = hb;
+  uint64_t hrow = (hb && before.lh) ? before.lh - 1 : base;
 #pragma unroll
   for (int j = 0; j < VT; ++j) {
     const uint64_t row = first + j;
@@ -326,6 +374,39 @@ __global__ __launch_bounds__(kBNT) void k_rbk_down(const K* __restrict__ keys,
       }
     }
   }
+}
+
+// One thread per tile: a tile whose first row continues a run that ends in
+// the tile holds that run's tile part; add the carry-in (the spine's
+// exclusive prefix: group carry, then the tiles before it in its group).
+template <class K, class V, class Op, int MODE>
+__global__ __launch_bounds__(256) void k_rbk_patch(const K* __restrict__ keys, Sizes z,
+                                                   uint32_t tile_rows,
+                                                   const Seg<V>* __restrict__ agg,
+                                                   const Seg<V>* __restrict__ ex,
+                                                   const Seg<V>* __restrict__ grp,
+                                                   V* __restrict__ out_vals,
+                                                   uint32_t* __restrict__ out_counts,
+                                                   uint64_t out_cap) {
+  const uint64_t n = rows_of(z);
+  const uint64_t T = (n + tile_rows - 1) / tile_rows;
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t == 0 || t >= T) return;
+  const uint64_t base = t * tile_rows;
+  if (!eq(keys[base - 1], keys[base])) return;  // the tile starts a run
+  // With no head in the tile, the open run ends here only at the last row
+  // or where the next tile starts a run.
+  const uint64_t next = base + tile_rows;
+  if (agg[t].h == 0 && next < n && eq(keys[next - 1], keys[next])) return;
+  const uint64_t g = t / kSpNT, l = t % kSpNT;
+  Seg<V> cin;
+  if (g == 0) cin = ex[t];
+  else if (l == 0) cin = grp[g];
+  else cin = seg_combine<Op>(grp[g], ex[t]);
+  const uint64_t r = static_cast<uint64_t>(cin.h) - 1;
+  if (r >= out_cap) return;
+  if constexpr (MODE & 1) out_vals[r] = Op::apply(cin.s, out_vals[r]);
+  if constexpr (MODE & 2) out_counts[r] += static_cast<uint32_t>(base - (cin.lh - 1));
 }
 
 // Sizes after the pass: the size variables may alias the row count, so
@@ -381,36 +462,48 @@ int rbk_impl(rfx_ctx* ctx, void* out_keys, void* out_vals, uint32_t* out_counts,
   }
   constexpr int TILE = RbkTune<K>::TILE;
   const uint64_t ntiles = (z.n_cap + TILE - 1) / TILE;
+  if (ntiles > 0xFFFFFFFFull) return fail(RFX_EINVAL, "reduce_by_key: %llu tiles",
+                                          static_cast<unsigned long long>(ntiles));
+  const uint64_t ngroups = (ntiles + kSpNT - 1) / kSpNT;
   Carve c(nullptr);
+  c.take<uint64_t>(1 + ntiles);
   c.take<Seg<V>>(ntiles);
+  c.take<Seg<V>>(ntiles);
+  c.take<Seg<V>>(ngroups);
   c.take<uint64_t>(1);
   RFX_TRY(ensure_workspace(ctx, c.off));
   Carve cw(ctx->ws);
+  uint64_t* st = cw.take<uint64_t>(1 + ntiles);
   Seg<V>* agg = cw.take<Seg<V>>(ntiles);
+  Seg<V>* ex = cw.take<Seg<V>>(ntiles);
+  Seg<V>* grp = cw.take<Seg<V>>(ngroups);
   uint64_t* runs = cw.take<uint64_t>(1);
+  RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(st), 2 * (1 + ntiles), 0));
   const bool vec = (reinterpret_cast<uintptr_t>(keys) % 16 == 0) &&
                    (!(MODE & 1) || reinterpret_cast<uintptr_t>(vals) % 16 == 0);
   const K* kp = static_cast<const K*>(keys);
   const V* vp = static_cast<const V*>(vals);
+  K* ok = static_cast<K*>(out_keys);
+  V* ov = static_cast<V*>(out_vals);
   const uint32_t g = static_cast<uint32_t>(ntiles);
   {
-    RFX_PROF(ctx, "rbk.up");
-    if (vec) k_rbk_up<K, V, Op, MODE, true><<<g, kBNT, 0, ctx->stream>>>(kp, vp, z, agg);
-    else k_rbk_up<K, V, Op, MODE, false><<<g, kBNT, 0, ctx->stream>>>(kp, vp, z, agg);
-    RFX_LAUNCHED();
-  }
-  k_rbk_spine<V, Op><<<1, kRChunks, 0, ctx->stream>>>(agg, z, TILE, runs);
-  RFX_LAUNCHED();
-  {
-    RFX_PROF(ctx, "rbk.down");
+    RFX_PROF(ctx, "rbk.pass");
     if (vec)
-      k_rbk_down<K, V, Op, MODE, true><<<g, kBNT, 0, ctx->stream>>>(
-          kp, vp, z, agg, static_cast<K*>(out_keys), static_cast<V*>(out_vals), out_counts, out_cap);
+      k_rbk_onepass<K, V, Op, MODE, true><<<g, kBNT, 0, ctx->stream>>>(kp, vp, z, st, agg, ok, ov,
+                                                                       out_counts, out_cap);
     else
-      k_rbk_down<K, V, Op, MODE, false><<<g, kBNT, 0, ctx->stream>>>(
-          kp, vp, z, agg, static_cast<K*>(out_keys), static_cast<V*>(out_vals), out_counts, out_cap);
+      k_rbk_onepass<K, V, Op, MODE, false><<<g, kBNT, 0, ctx->stream>>>(kp, vp, z, st, agg, ok, ov,
+                                                                        out_counts, out_cap);
     RFX_LAUNCHED();
   }
+  k_rbk_spine_group<V, Op><<<static_cast<uint32_t>(ngroups), kSpNT, 0, ctx->stream>>>(agg, ex, z,
+                                                                                    TILE, grp);
+  RFX_LAUNCHED();
+  k_rbk_spine<V, Op><<<1, kRChunks, 0, ctx->stream>>>(grp, z, TILE, runs);
+  RFX_LAUNCHED();
+  k_rbk_patch<K, V, Op, MODE><<<static_cast<uint32_t>((ntiles + 255) / 256), 256, 0, ctx->stream>>>(
+      kp, z, TILE, agg, ex, grp, ov, out_counts, out_cap);
+  RFX_LAUNCHED();
   k_rbk_sizes<<<1, 64, 0, ctx->stream>>>(z, runs, z.n_cap);
   RFX_LAUNCHED();
   return RFX_OK;
