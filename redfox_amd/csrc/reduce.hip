@@ -268,11 +268,10 @@ __global__ __launch_bounds__(kSpNT) void k_rbk_spine_group(const Seg<V>* agg, Se
 
 // One block of kRChunks threads: exclusive segmented scan of the group
 // totals in place (thread t folds groups [t*C, t*C + C) in order, the block
-// scan combines the threads; C depends on n only), and the run count.
+// scan combines the threads; C depends on n only).
 template <class V, class Op>
 __global__ __launch_bounds__(kRChunks) void k_rbk_spine(Seg<V>* __restrict__ grp, Sizes z,
-                                                        uint32_t tile_rows,
-                                                        uint64_t* __restrict__ runs) {
+                                                        uint32_t tile_rows) {
   __shared__ SegE<V, Op> s_wave[kRChunks / kWave];
   const uint32_t tid = threadIdx.x;
   const uint64_t n = rows_of(z);
@@ -288,7 +287,6 @@ __global__ __launch_bounds__(kRChunks) void k_rbk_spine(Seg<V>* __restrict__ grp
     grp[t] = carry.e ? Seg<V>{0, V{}, 0} : carry.s;
     carry = sege_combine(carry, SegE<V, Op>{a, false});
   }
-  if (tid == 0) *runs = total.e ? 0 : total.s.h;
 }
 
 // ---- one pass over the rows, then a patch of one run per tile -------------
@@ -316,7 +314,9 @@ __global__ __launch_bounds__(kBNT) void k_rbk_onepass(
   __shared__ Seg<V> s_wave[kBNT / kWave];
   __shared__ uint64_t s_hx;
   __shared__ uint32_t s_tile;
-  uint64_t* hst = st + 1;  // st[0]: ticket (low half); hst[t]: head-count word of tile t
+  // st[0]: ticket (low half); st[1]: run count, from the last tile; st[2 + t]:
+  // head-count word of tile t.
+  uint64_t* hst = st + 2;
   if (threadIdx.x == 0) s_tile = atomicAdd(reinterpret_cast<uint32_t*>(st), 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
@@ -339,9 +339,11 @@ __global__ __launch_bounds__(kBNT) void k_rbk_onepass(
       if (threadIdx.x == 0) {
         status_store64(hst + tile, kFlagIncl64 | (hx + total.h));
         s_hx = hx;
+        if (base + TILE >= n) st[1] = hx + total.h;
       }
     } else if (threadIdx.x == 0) {
       s_hx = 0;
+      if (base + TILE >= n) st[1] = total.h;
     }
   }
   __syncthreads();
@@ -466,19 +468,18 @@ int rbk_impl(rfx_ctx* ctx, void* out_keys, void* out_vals, uint32_t* out_counts,
                                           static_cast<unsigned long long>(ntiles));
   const uint64_t ngroups = (ntiles + kSpNT - 1) / kSpNT;
   Carve c(nullptr);
-  c.take<uint64_t>(1 + ntiles);
+  c.take<uint64_t>(2 + ntiles);
   c.take<Seg<V>>(ntiles);
   c.take<Seg<V>>(ntiles);
   c.take<Seg<V>>(ngroups);
-  c.take<uint64_t>(1);
   RFX_TRY(ensure_workspace(ctx, c.off));
   Carve cw(ctx->ws);
-  uint64_t* st = cw.take<uint64_t>(1 + ntiles);
+  uint64_t* st = cw.take<uint64_t>(2 + ntiles);
   Seg<V>* agg = cw.take<Seg<V>>(ntiles);
   Seg<V>* ex = cw.take<Seg<V>>(ntiles);
   Seg<V>* grp = cw.take<Seg<V>>(ngroups);
-  uint64_t* runs = cw.take<uint64_t>(1);
-  RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(st), 2 * (1 + ntiles), 0));
+  uint64_t* runs = st + 1;  // zero unless a tile ran
+  RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(st), 2 * (2 + ntiles), 0));
   const bool vec = (reinterpret_cast<uintptr_t>(keys) % 16 == 0) &&
                    (!(MODE & 1) || reinterpret_cast<uintptr_t>(vals) % 16 == 0);
   const K* kp = static_cast<const K*>(keys);
@@ -496,14 +497,21 @@ int rbk_impl(rfx_ctx* ctx, void* out_keys, void* out_vals, uint32_t* out_counts,
                                                                         out_counts, out_cap);
     RFX_LAUNCHED();
   }
-  k_rbk_spine_group<V, Op><<<static_cast<uint32_t>(ngroups), kSpNT, 0, ctx->stream>>>(agg, ex, z,
-                                                                                    TILE, grp);
-  RFX_LAUNCHED();
-  k_rbk_spine<V, Op><<<1, kRChunks, 0, ctx->stream>>>(grp, z, TILE, runs);
-  RFX_LAUNCHED();
-  k_rbk_patch<K, V, Op, MODE><<<static_cast<uint32_t>((ntiles + 255) / 256), 256, 0, ctx->stream>>>(
-      kp, z, TILE, agg, ex, grp, ov, out_counts, out_cap);
-  RFX_LAUNCHED();
+  // Carry-ins: only runs that cross tiles need them, so one tile needs none,
+  // and one group needs no scan of group totals.
+  if (ntiles > 1) {
+    k_rbk_spine_group<V, Op><<<static_cast<uint32_t>(ngroups), kSpNT, 0, ctx->stream>>>(
+        agg, ex, z, TILE, grp);
+    RFX_LAUNCHED();
+    if (ngroups > 1) {
+      k_rbk_spine<V, Op><<<1, kRChunks, 0, ctx->stream>>>(grp, z, TILE);
+      RFX_LAUNCHED();
+    }
+    k_rbk_patch<K, V, Op, MODE><<<static_cast<uint32_t>((ntiles + 255) / 256), 256, 0,
+                                  ctx->stream>>>(kp, z, TILE, agg, ex, grp, ov, out_counts,
+                                                 out_cap);
+    RFX_LAUNCHED();
+  }
   k_rbk_sizes<<<1, 64, 0, ctx->stream>>>(z, runs, z.n_cap);
   RFX_LAUNCHED();
   return RFX_OK;
