@@ -139,6 +139,26 @@ def test_runs_across_scan_groups(rf, ctx, orc, kb):
     assert np.array_equal(gk, ok) and gc.tolist() == lengths
 
 
+@pytest.mark.parametrize("kind,op", [("f64", 1), ("f64", 2), ("u32", 0)])
+def test_long_runs_exact_ops(rf, ctx, orc, kind, op):
+    """MIN, MAX and u32 SUM of runs spanning many tiles and scan groups
+    (the carry-in of a run open at a tile start is combined with the tile's
+    part afterwards): bit-exact against the oracle."""
+    n = 20_000_005
+    keys = np.repeat(np.arange(6, dtype=np.uint32) * np.uint32(5),
+                     [1, 3_000_000, 8_388_608, 5, 8_611_388, 3]).astype(np.uint32)
+    assert keys.size == n
+    if kind == "u32":
+        vals = datagen.u32(61, n)
+        gk, gv = gpu_rbk(rf, ctx, keys, vals, 4, kind="u32")
+        ok, ov = orc.reduce_by_key_u32(keys, vals, 4)
+    else:
+        vals = datagen.uniform01(62 + op, n) * 2.0 - 1.0
+        gk, gv = gpu_rbk(rf, ctx, keys, vals, 4, op)
+        ok, ov = orc.reduce_by_key_f64(keys, vals, 4, op)
+    assert np.array_equal(gk, ok) and np.array_equal(gv, ov)
+
+
 def test_runs_not_sets(rf, ctx):
     keys = np.array([1, 1, 2, 1, 1], dtype=np.uint8)
     vals = np.array([1.0, 2.0, 3.0, 4.0, 5.0])
