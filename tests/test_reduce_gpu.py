@@ -78,7 +78,7 @@ def test_u32_sum_and_count(rf, ctx, orc):
     assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
 
 
-@pytest.mark.parametrize("n", [1, 2, 2047, 8191, 8192, 8193, 16385, 8192 * 1024 + 5, 5_000_000])
+@pytest.mark.parametrize("n", [1, 2, 2047, 12287, 12288, 12289, 24577, 12288 * 1024 + 5, 5_000_000])
 def test_sizes_and_long_runs(rf, ctx, orc, n):
     # Few keys: runs span many tiles and chunks.
     keys = sorted_keys(orc, 1, n, n, 3)
@@ -92,9 +92,9 @@ def test_sizes_and_long_runs(rf, ctx, orc, n):
 @pytest.mark.parametrize("kb", [4, 16])
 def test_run_ends_on_tile_boundaries(rf, ctx, orc, kb):
     """Runs ending exactly at, just before and just after tile boundaries
-    (tiles hold 8192 rows, 4096 for 16-byte keys), and runs spanning many
+    (tiles hold 12288 rows, 6144 for 16-byte keys), and runs spanning many
     tiles."""
-    T = 8192 if kb == 4 else 4096
+    T = 12288 if kb == 4 else 6144
     lengths = [T, 3 * T, T - 1, 1, T + 1, 2 * T - 1, 2, 70 * T, 2 * T + 1, 5, T]
     n = sum(lengths)
     run_keys = np.arange(len(lengths), dtype=np.uint64) * np.uint64(7) + np.uint64(3)
@@ -116,10 +116,10 @@ def test_run_ends_on_tile_boundaries(rf, ctx, orc, kb):
 
 @pytest.mark.parametrize("kb", [4, 16])
 def test_runs_across_scan_groups(rf, ctx, orc, kb):
-    """Tile aggregates are scanned in groups of 1024 tiles (8192 rows a
-    tile, 4096 for 16-byte keys): runs ending at, just before and just after
+    """Tile aggregates are scanned in groups of 1024 tiles (12288 rows a
+    tile, 6144 for 16-byte keys): runs ending at, just before and just after
     group boundaries, runs spanning whole groups, and a short last group."""
-    G = 1024 * (8192 if kb == 4 else 4096)
+    G = 1024 * (12288 if kb == 4 else 6144)
     lengths = [G - 1, 1, G, 5, 2 * G + 3, 4096, G - 4096 - 9, 1, 1, 7, G + 1, 333]
     n = sum(lengths)
     run_keys = np.arange(len(lengths), dtype=np.uint64) * np.uint64(11) + np.uint64(2)
