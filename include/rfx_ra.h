@@ -263,7 +263,9 @@ int rfx_join_expand_project(rfx_ctx* ctx, const int32_t* lower_bound,
 /* Reduce-by-key over runs of equal adjacent keys (reduce_by_key in
  * Reduce.cu:175-1532).  keys/vals hold *n rows (device u64 *n);
  * out_keys/out_vals receive one row per run and *n_out (device u64) the
- * run count.  op: RFX_AGG_SUM, RFX_AGG_MIN, RFX_AGG_MAX. */
+ * run count.  op: RFX_AGG_SUM, RFX_AGG_MIN, RFX_AGG_MAX.  Outputs must
+ * not overlap keys or vals (as for thrust::reduce_by_key); an output that
+ * is an input gives RFX_EINVAL. */
 #define RFX_AGG_SUM 0
 #define RFX_AGG_MIN 1
 #define RFX_AGG_MAX 2

@@ -563,6 +563,17 @@ int reduce_by_key_sized(rfx_ctx* ctx, int op, int val_kind, int kb, void* out_ke
                 (val_kind == 3 && !out_counts)))
     return fail(RFX_EINVAL, "reduce_by_key: null column");
   if (n_cap >= (1ull << 31)) return fail(RFX_ERANGE, "reduce_by_key: too many rows");
+  if (n_cap) {
+    // No output may be an input (as for thrust::reduce_by_key): tiles read
+    // their neighbours' keys, and the patch reads keys again after every
+    // run is written.  n_cap bounds the rows but not the buffers, so only
+    // the in-place case is detectable here.
+    const void* outs[3] = {out_keys, out_vals, val_kind == 3 ? out_counts : nullptr};
+    const void* ins[2] = {keys, val_kind == 2 ? nullptr : vals};
+    for (const void* o : outs)
+      for (const void* i : ins)
+        if (o && i && o == i) return fail(RFX_EINVAL, "reduce_by_key: an output is an input");
+  }
   if (val_kind == 0 || val_kind == 3) {  // f64
     if (val_kind == 3) {
       switch (op) {

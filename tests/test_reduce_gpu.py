@@ -159,6 +159,26 @@ def test_long_runs_exact_ops(rf, ctx, orc, kind, op):
     assert np.array_equal(gk, ok) and np.array_equal(gv, ov)
 
 
+def test_in_place_output_rejected(rf, ctx):
+    """An output that is the keys or vals buffer is refused before any
+    launch."""
+    n = 1000
+    K = rf.DeviceBuffer.from_numpy(np.arange(2 * n, dtype=np.uint32))
+    V = rf.DeviceBuffer.from_numpy(np.ones(n))
+    OV = rf.DeviceBuffer(8 * n)
+    nout, ncell = rf.u64_cell(), rf.u64_cell(n)
+    with pytest.raises(rf.RfxError) as e:
+        ctx.reduce_by_key_f64(K, OV, nout, K, V, ncell, n, 4, 0)
+    assert e.value.status == -1
+    with pytest.raises(rf.RfxError):
+        ctx.count_by_key(OV, K, nout, K, ncell, n, 4)
+    # Disjoint buffers still run.
+    OK = rf.DeviceBuffer(4 * n)
+    ctx.reduce_by_key_f64(OK, OV, nout, K, V, ncell, n, 4, 0)
+    ctx.synchronize()
+    assert rf.read_u64(nout) == n
+
+
 def test_runs_not_sets(rf, ctx):
     keys = np.array([1, 1, 2, 1, 1], dtype=np.uint8)
     vals = np.array([1.0, 2.0, 3.0, 4.0, 5.0])
