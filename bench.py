@@ -43,6 +43,7 @@ def median(xs):
 
 COPY_GBPS = None  # measured device-to-device copy rate (read + write bytes)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sort_onesweep_2p28_u32_traffic.txt")
+AGG_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "reduce_by_key_2p28_zipf_traffic.txt")
 
 
 def roofline(bytes_per_launch, launches, total_ms, traffic=None):
@@ -75,12 +76,12 @@ def copy_bandwidth(rf, ctx, nbytes=1 << 30, reps=5):
     return 2 * nbytes / (median(times) * 1e-3) / 1e9
 
 
-def measured_traffic():
-    """HBM bytes per onesweep launch from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on the 1 GiB
-    copy), or None."""
+def measured_traffic(path=TRAFFIC_FILE):
+    """HBM bytes per launch of a kernel (the onesweep pass by default) from
+    its committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE, scaled by
+    the same counters on the 1 GiB copy), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return int(line.split(":", 1)[1])
@@ -442,7 +443,7 @@ def bench_agg(rf, ctx, iters=5):
     cnt = oc.download(np.uint32, g)
     res = {"n": n, "groups": g, "ms": median(times), "rows_per_s": n / (median(times) * 1e-3),
            "counts_sum_to_n": bool(int(cnt.sum(dtype=np.uint64)) == n),
-           "roofline": roofline(12 * n + 16 * g, lo, mo)}
+           "roofline": roofline(12 * n + 16 * g, lo, mo, measured_traffic(AGG_TRAFFIC_FILE))}
     for b in (keys, vals, ident, row_off, dim_off, ok, ov, oc, nout, ncell):
         b.close()
     return res
