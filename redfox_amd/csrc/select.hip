@@ -22,11 +22,12 @@ namespace {
 #define RFX_SEL_NT 512
 #endif
 #ifndef RFX_SEL_VT16
-#define RFX_SEL_VT16 16
+#define RFX_SEL_VT16 24
 #endif
 constexpr int kFNT = RFX_SEL_NT;
 constexpr int kFNW = kFNT / kWave;
-// Rows per thread: 16, or fewer for 16-byte tuples (register budget).
+// Rows per thread: 16, and 24 for 16-byte tuples (2^27 tuples, half kept:
+// 0.64 ms against 0.69 ms with 16 and 0.75 ms with 32).
 template <class K>
 constexpr int kFVT = sizeof(K) >= 16 ? RFX_SEL_VT16 : 16;
 
