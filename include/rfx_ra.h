@@ -141,7 +141,11 @@ int rfx_gen_lineitem(rfx_ctx* ctx, int32_t* shipdate, uint8_t* returnflag,
 /* Streams and graphs: a context bound to its own stream can record a whole
  * plan (any sequence of native operators) into a HIP graph and replay it
  * with one launch.  Reserve the workspace before capturing: growing it
- * during capture fails with RFX_EINVAL.  Profiling must be off. */
+ * during capture fails with RFX_EINVAL.  A graph keeps the workspace's
+ * addresses, so while any graph from rfx_graph_end is alive (until
+ * rfx_graph_destroy) an operator, tuning change or rfx_ctx_reserve that
+ * would grow the workspace fails with RFX_EINVAL instead of freeing it.
+ * Destroy a context's graphs before the context.  Profiling must be off. */
 int rfx_stream_create(void** stream);
 int rfx_stream_destroy(void* stream);
 int rfx_stream_synchronize(void* stream);
@@ -264,8 +268,9 @@ int rfx_join_expand_project(rfx_ctx* ctx, const int32_t* lower_bound,
  * Reduce.cu:175-1532).  keys/vals hold *n rows (device u64 *n);
  * out_keys/out_vals receive one row per run and *n_out (device u64) the
  * run count.  op: RFX_AGG_SUM, RFX_AGG_MIN, RFX_AGG_MAX.  Outputs must
- * not overlap keys or vals (as for thrust::reduce_by_key); an output that
- * is an input gives RFX_EINVAL. */
+ * not overlap keys or vals (as for thrust::reduce_by_key): an output whose
+ * first min(n_cap, out_cap) rows overlap the inputs' first n_cap rows
+ * gives RFX_EINVAL. */
 #define RFX_AGG_SUM 0
 #define RFX_AGG_MIN 1
 #define RFX_AGG_MAX 2

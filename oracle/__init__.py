@@ -1,5 +1,4 @@
-"""ctypes binding of the CPU oracle (librfx_oracle.so) and of the reference's
-own tuple codec built into _ref/ (when it was built).
+"""ctypes binding of the CPU oracle (librfx_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, and only as the checker or the reported CPU
@@ -15,7 +14,6 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "librfx_oracle.so")
-REF_TUPLE = os.path.join(HERE, "_ref", "libref_tuple.so")
 
 P, U64, U32, I = c_void_p, c_uint64, c_uint32, c_int
 
@@ -62,19 +60,6 @@ def lib() -> ctypes.CDLL:
             fn.restype = res
             fn.argtypes = args
     return _lib
-
-
-def ref_tuple():
-    """The reference's Tuple.h codec, or None if oracle/_ref was not built."""
-    if not os.path.exists(REF_TUPLE):
-        return None
-    L = ctypes.CDLL(REF_TUPLE)
-    L.ref_extract.restype = U64
-    L.ref_extract.argtypes = [I, P, I]
-    L.ref_insert.argtypes = [I, P, I, U64]
-    for n in ("ref_strip_values", "ref_strip_keys", "ref_restore_values"):
-        getattr(L, n).argtypes = [I, I, P, P]
-    return L
 
 
 def _p(a):

@@ -2,10 +2,26 @@
 // streams and HIP-graph capture of a context's work, segmented (instance-
 // prefixed) keys, and the synthetic fact/dimension generator of the
 // batched join-aggregate workload.
+#include <map>
+#include <mutex>
+
 #include "rfx_device.h"
 #include "rfx_internal.h"
 
 namespace rfx {
+
+// Instantiated graphs and the context whose workspace they captured: while
+// a context owns live graphs its workspace may neither grow nor be freed
+// (the graphs hold its addresses).
+static std::mutex g_graph_mu;
+static std::map<void*, rfx_ctx*> g_graph_owner;
+
+void forget_graphs_of(rfx_ctx* ctx) {
+  std::lock_guard<std::mutex> lk(g_graph_mu);
+  for (auto it = g_graph_owner.begin(); it != g_graph_owner.end();)
+    it = it->second == ctx ? g_graph_owner.erase(it) : std::next(it);
+}
+
 namespace {
 
 __device__ __forceinline__ uint64_t mix(uint64_t seed, uint64_t i) {
@@ -134,6 +150,11 @@ int rfx_graph_end(rfx_ctx* ctx, void** graph_exec) {
   if (err != hipSuccess)
     return fail(RFX_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(err));
   *graph_exec = e;
+  {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    g_graph_owner[e] = ctx;
+  }
+  ctx->live_graphs += 1;
   return RFX_OK;
 }
 
@@ -144,7 +165,16 @@ int rfx_graph_launch(rfx_ctx* ctx, void* graph_exec) {
 }
 
 int rfx_graph_destroy(void* graph_exec) {
-  if (graph_exec) RFX_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)));
+  if (!graph_exec) return RFX_OK;
+  {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    auto it = g_graph_owner.find(graph_exec);
+    if (it != g_graph_owner.end()) {
+      it->second->live_graphs -= 1;
+      g_graph_owner.erase(it);
+    }
+  }
+  RFX_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)));
   return RFX_OK;
 }
 

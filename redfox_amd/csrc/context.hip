@@ -31,6 +31,11 @@ int ensure_workspace(rfx_ctx* ctx, uint64_t bytes) {
   if (ctx->capturing)
     return fail(RFX_EINVAL, "workspace must grow to %llu bytes during graph capture: reserve first",
                 static_cast<unsigned long long>(bytes));
+  if (ctx->live_graphs > 0)
+    return fail(RFX_EINVAL,
+                "workspace must grow to %llu bytes while %d captured graph(s) of this context "
+                "still use it: destroy them first, or reserve before capturing",
+                static_cast<unsigned long long>(bytes), ctx->live_graphs);
   if (ctx->ws_borrowed)
     return fail(RFX_EBUDGET, "operator needs %llu workspace bytes, the borrowed region holds %llu",
                 static_cast<unsigned long long>(bytes),
@@ -245,6 +250,7 @@ int rfx_ctx_create(rfx_ctx** out, int device, void* stream) {
 
 int rfx_ctx_destroy(rfx_ctx* ctx) {
   if (!ctx) return RFX_OK;
+  forget_graphs_of(ctx);
   (void)prof_collect(ctx);
   for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
   if (ctx->ws && !ctx->ws_borrowed) {

@@ -169,7 +169,7 @@ struct TileRows {
 
 // Block scan of the thread aggregates: *before = combination of the
 // threads before this one (have_before false for thread 0), returns the
-// tile total.  Threads past the last row hold h = 0.
+// tile total.  Threads past the last row hold h = 0 and the identity.
 template <class V, class Op>
 __device__ __forceinline__ Seg<V> tile_scan(const Seg<V>& mine, Seg<V>* s_wave, Seg<V>* before,
                                             bool* have_before) {
@@ -330,7 +330,11 @@ __global__ __launch_bounds__(kBNT) void k_rbk_onepass(
   t.load(keys, vals, n, base, first, s_first, s_last);
   Seg<V> before;
   bool hb;
-  const Seg<V> total = tile_scan<V, Op>(t.mine, s_wave, &before, &hb);
+  // Threads past the last row carry the operator's identity, so the tile
+  // aggregate is exact for MIN and MAX as well.
+  Seg<V> mine = t.mine;
+  if (!t.any) mine.s = Op::template identity<V>();
+  const Seg<V> total = tile_scan<V, Op>(mine, s_wave, &before, &hb);
   if (threadIdx.x < kWave) {
     if (threadIdx.x == 0) {
       agg[tile] = total;
@@ -564,15 +568,23 @@ int reduce_by_key_sized(rfx_ctx* ctx, int op, int val_kind, int kb, void* out_ke
     return fail(RFX_EINVAL, "reduce_by_key: null column");
   if (n_cap >= (1ull << 31)) return fail(RFX_ERANGE, "reduce_by_key: too many rows");
   if (n_cap) {
-    // No output may be an input (as for thrust::reduce_by_key): tiles read
-    // their neighbours' keys, and the patch reads keys again after every
-    // run is written.  n_cap bounds the rows but not the buffers, so only
-    // the in-place case is detectable here.
-    const void* outs[3] = {out_keys, out_vals, val_kind == 3 ? out_counts : nullptr};
-    const void* ins[2] = {keys, val_kind == 2 ? nullptr : vals};
-    for (const void* o : outs)
-      for (const void* i : ins)
-        if (o && i && o == i) return fail(RFX_EINVAL, "reduce_by_key: an output is an input");
+    // No output may overlap an input (as for thrust::reduce_by_key): tiles
+    // read their neighbours' keys, and the patch reads keys again after
+    // every run is written.  The inputs span n_cap rows; the outputs at
+    // most min(n_cap, out_cap) runs.
+    const uint64_t vb = val_kind == 1 || val_kind == 2 ? 4 : 8;
+    const uint64_t runs = out_cap < n_cap ? out_cap : n_cap;
+    struct Span { const void* p; uint64_t bytes; };
+    const Span outs[3] = {{out_keys, runs * kb}, {out_vals, runs * vb},
+                          {val_kind == 3 ? out_counts : nullptr, runs * 4}};
+    const Span ins[2] = {{keys, n_cap * kb}, {val_kind == 2 ? nullptr : vals, n_cap * vb}};
+    for (const Span& o : outs)
+      for (const Span& i : ins) {
+        if (!o.p || !i.p || !o.bytes) continue;
+        const uintptr_t o0 = reinterpret_cast<uintptr_t>(o.p), i0 = reinterpret_cast<uintptr_t>(i.p);
+        if (o0 < i0 + i.bytes && i0 < o0 + o.bytes)
+          return fail(RFX_EINVAL, "reduce_by_key: an output overlaps an input");
+      }
   }
   if (val_kind == 0 || val_kind == 3) {  // f64
     if (val_kind == 3) {

@@ -26,6 +26,7 @@ struct rfx_ctx {
   bool profiling = false;
   bool capturing = false;     // inside rfx_graph_begin/end: no allocation
   bool ws_borrowed = false;   // ws belongs to the caller: never grown or freed
+  int live_graphs = 0;        // instantiated graphs that hold pointers into ws
   // Tile shapes chosen by rfx_ctx_tune (0: the built-in default).
   int tune_sort_ipt = 0;      // keys per thread of the onesweep tile
   int tune_fb_tile = 0;       // merged keys per find_bounds tile
@@ -66,7 +67,10 @@ inline bool valid_width(int b) {
 inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 // Make sure ctx->ws holds `bytes` (synchronises the stream before growing).
+// Refused while graphs captured on the context are alive.
 int ensure_workspace(rfx_ctx* ctx, uint64_t bytes);
+// Drop the graph-ownership records of a context being destroyed.
+void forget_graphs_of(rfx_ctx* ctx);
 
 // A bump allocator over a workspace region.  Constructed with a null base
 // it only measures: `off` ends as the bytes the same sequence needs.

@@ -572,7 +572,7 @@ class Plan:
                 raise ValueError("columns of one relation have one length")
             b = self._buf(a.nbytes)
             if n:
-                b.upload(a)
+                self._upload(b, a)
             cols.append((b, t))
             host.append(a)
         lead = types.index("str") if "str" in types else len(types)
@@ -615,8 +615,16 @@ class Plan:
 
     def _cell(self, value: int = 0):
         c = self._buf(8)
-        c.upload(np.array([value], dtype=np.uint64))
+        self._upload(c, np.array([value], dtype=np.uint64))
         return c
+
+    def _upload(self, buf, arr: np.ndarray) -> None:
+        """Host write into a (possibly pooled) buffer.  The copy is a
+        blocking host-to-device copy, which is not ordered with the
+        context's stream, so work queued there that may still read the
+        buffer's previous contents is waited for first."""
+        self.ctx.synchronize()
+        buf.upload(arr)
 
     def _release(self, *bufs) -> None:
         """Return buffers to the pool (work already queued on the stream
@@ -711,7 +719,7 @@ class Plan:
         pat_bufs, pat_cols = [], {}
         for k, pat in enumerate(pats):
             text = self._buf(len(pat) + 1)
-            text.upload(np.frombuffer(pat + b"\0", dtype=np.uint8))
+            self._upload(text, np.frombuffer(pat + b"\0", dtype=np.uint8))
             cell = self._cell(text.ptr)
             pat_bufs += [text, cell]
             pat_cols.setdefault(pat, len(src.cols) + k)
@@ -961,7 +969,7 @@ class Plan:
         cols = []
         for (v, _), t in zip(vals, types):
             b = self._buf(8)
-            b.upload(np.array([v], dtype=_NP[t]))
+            self._upload(b, np.array([v], dtype=_NP[t]))
             cols.append((b, t))
         return Relation(cols, 1, len(types))
 

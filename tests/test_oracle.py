@@ -1,11 +1,11 @@
 """Pins the CPU oracle before it is trusted as the GPU checker.
 
 Each oracle function is checked against (a) the reference's own fixtures and
-known answers (TPC-H/data/A from TC.tbl, TestJoin's identity join), (b) the
-reference's own tuple codec compiled from its sources into oracle/_ref, and
-(c) independent formulations that a plausible slip (dropped term, wrong tie
+known answers (TPC-H/data/A from TC.tbl, TestJoin's identity join), and (b)
+independent formulations that a plausible slip (dropped term, wrong tie
 rule, off-by-one, swapped operand) would fail: Python's stable sort, nested
-loop joins, collections.Counter multisets, math.fsum.
+loop joins, collections.Counter multisets, math.fsum, the tuple as a bit
+string.
 """
 import collections
 import math
@@ -79,55 +79,69 @@ def test_tuple_closed_forms(orc):
     assert L.rfxo_tuple_bytes(b3.ctypes.data, 2) == 1
 
 
-@pytest.fixture(scope="module")
-def ref():
-    import oracle
-
-    r = oracle.ref_tuple()
-    if r is None:
-        pytest.skip("oracle/_ref not built (no reference checkout at build time)")
-    return r
+# Tuple shapes: config-1 rows, join output, Q1 aggregate output, and words
+# of 8, 32, 64 and 128 bits with odd and word-crossing field widths.
+SHAPES = [(32, 32), (32, 32, 32), (2, 1, 64), (16, 8, 8), (3, 23, 38), (64, 64), (5, 3),
+          (1, 40, 40), (7,)]
 
 
-def test_codec_matches_reference_tuple_h(orc, ref):
-    """extract / insert / stripValues / stripKeys / restoreValues of the
-    reference's Tuple.h against the oracle's codec, every field of every
-    shape, random words."""
+def layout_bits(bits):
+    """The tuple as a bit string, most significant bit first: padding up
+    to the word (the smallest of 8..128 bits holding the fields,
+    Tuple.h:30-37), then field 0, field 1, ... (Tuple.h:718-790).  A
+    formulation by string slicing, independent of the oracle's shifts."""
+    used = sum(bits)
+    word = next(w for w in (8, 16, 32, 64, 128) if used <= w)
+    spans, at = [], word - used
+    for b in bits:
+        spans.append((at, at + b))
+        at += b
+    return word, spans
+
+
+def test_codec_matches_bit_string_layout(orc):
+    """extract / insert of every field, and the split (stripValues /
+    stripKeys) and merge (restoreValues) forms, against slices of the
+    tuple's bit string."""
     L = orc.lib()
-    shapes = ref.ref_shape_count()
-    words = datagen.splitmix64(77, 2 * 64).reshape(64, 2)
-    for s in range(shapes):
-        nf = ref.ref_shape_fields(s)
-        bits = np.array([ref.ref_shape_bits(s, f) for f in range(nf)], dtype=np.uint32)
-        nbytes = ref.ref_shape_bytes(s)
-        assert L.rfxo_tuple_bytes(bits.ctypes.data, nf) == nbytes
+    words = datagen.splitmix64(77, 2 * 48).reshape(48, 2)
+    for bits in SHAPES:
+        word, spans = layout_bits(bits)
+        b = np.array(bits, dtype=np.uint32)
+        nf = len(bits)
+        assert L.rfxo_tuple_bytes(b.ctypes.data, nf) == word // 8
         for w0 in words:
-            w = w0.copy()
-            # keep the word within its type and the used bits
-            if nbytes < 16:
-                w[1] = 0
-                if nbytes < 8:
-                    w[0] &= np.uint64((1 << (8 * nbytes)) - 1)
-            for f in range(nf):
-                assert ref.ref_extract(s, w.ctypes.data, f) == \
-                    L.rfxo_tuple_extract(w.ctypes.data, bits.ctypes.data, nf, f), (s, f)
-                v = int(datagen.splitmix64(int(w[0]) & 0xFFFF, 1)[0])
-                a, b = w.copy(), w.copy()
-                ref.ref_insert(s, a.ctypes.data, f, v)
-                L.rfxo_tuple_insert(b.ctypes.data, bits.ctypes.data, nf, f, v)
-                assert np.array_equal(a, b), (s, f)
-            for keys in (1, 2):
-                if keys >= nf:
-                    continue
-                shift = int(L.rfxo_bits_after(bits.ctypes.data, nf, keys - 1))
-                val = ints(w, 16)[0]
-                o = np.zeros(2, dtype=np.uint64)
-                ref.ref_strip_values(s, keys, w.ctypes.data, o.ctypes.data)
-                assert ints(o, 16)[0] == val >> shift
-                ref.ref_strip_keys(s, keys, w.ctypes.data, o.ctypes.data)
-                assert ints(o, 16)[0] == val & ((1 << shift) - 1)
-                ref.ref_restore_values(s, keys, w.ctypes.data, o.ctypes.data)
-                assert ints(o, 16)[0] == (val << shift) & ((1 << (8 * nbytes)) - 1)
+            val = (int(w0[0]) | (int(w0[1]) << 64)) & ((1 << word) - 1)
+            s = format(val, f"0{word}b")
+            w = from_ints([val], 16)
+            for f, (lo, hi) in enumerate(spans):
+                assert L.rfxo_tuple_extract(w.ctypes.data, b.ctypes.data, nf, f) == \
+                    int(s[lo:hi], 2) % (1 << 64), (bits, f)
+                if bits[f] <= 64:
+                    v = int(datagen.splitmix64(val & 0xFFFF, 1)[0])
+                    a = w.copy()
+                    L.rfxo_tuple_insert(a.ctypes.data, b.ctypes.data, nf, f, v)
+                    fb = format(v % (1 << bits[f]), f"0{bits[f]}b")
+                    assert ints(a, 16)[0] == int(s[:lo] + fb + s[hi:], 2), (bits, f)
+            for keys in range(1, nf):
+                cut = spans[keys - 1][1]  # the key fields end here
+                shift = int(L.rfxo_bits_after(b.ctypes.data, nf, keys - 1))
+                assert shift == word - cut
+                # split of the tuple (stripValues, stripKeys) and merge
+                # (restoreValues | values) by the oracle, against slices.
+                w8 = from_ints([val], 16)
+                kb, vb = (cut + 7) // 8, (word - cut + 7) // 8
+                kb = next(x for x in (1, 2, 4, 8, 16) if x >= kb)
+                vb = next(x for x in (1, 2, 4, 8, 16) if x >= vb)
+                if word == 128:
+                    k, v = orc.split(w8, 16, kb, vb, shift)
+                else:
+                    k, v = orc.split(w8[:1].view(np.uint64).astype(DT[word // 8]), word // 8,
+                                     kb, vb, shift)
+                assert ints(k, kb)[0] == int(s[:cut], 2)
+                assert ints(v, vb)[0] == int(s[cut:], 2)
+                back = orc.merge(word // 8, k, kb, v, vb, shift)
+                assert ints(back, word // 8)[0] == val
 
 
 def test_split_merge_roundtrip(orc):

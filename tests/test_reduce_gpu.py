@@ -172,9 +172,26 @@ def test_in_place_output_rejected(rf, ctx):
     assert e.value.status == -1
     with pytest.raises(rf.RfxError):
         ctx.count_by_key(OV, K, nout, K, ncell, n, 4)
-    # Disjoint buffers still run.
+
+    class At:  # a view into a buffer at a byte offset
+        def __init__(self, buf, off):
+            self.ptr = buf.ptr + off
+
+    # Partial overlaps: out_keys inside the key rows, out_vals straddling
+    # the end of the value rows.
+    with pytest.raises(rf.RfxError):
+        ctx.reduce_by_key_f64(At(K, 4 * 7), OV, nout, K, V, ncell, n, 4, 0)
+    BIG = rf.DeviceBuffer(16 * n)
+    with pytest.raises(rf.RfxError):
+        ctx.reduce_by_key_f64(At(BIG, 8 * n), At(BIG, 8 * n - 8), nout, K, At(BIG, 0),
+                              ncell, n, 4, 0)
+    # Disjoint buffers still run, also when they are adjacent.
     OK = rf.DeviceBuffer(4 * n)
     ctx.reduce_by_key_f64(OK, OV, nout, K, V, ncell, n, 4, 0)
+    ctx.synchronize()
+    assert rf.read_u64(nout) == n
+    BIG.upload(np.ones(n))
+    ctx.reduce_by_key_f64(OK, At(BIG, 8 * n), nout, K, At(BIG, 0), ncell, n, 4, 0)
     ctx.synchronize()
     assert rf.read_u64(nout) == n
 
