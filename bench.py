@@ -224,6 +224,9 @@ def bench_join(rf, ctx, n_r, n_s, key_bits, iters):
     # r_val per left row, s_val per output (gathered), 16 B out per output.
     eg_bytes = 12 * n_r + 4 * total + 16 * total
     res["expand_gather_roofline"] = roofline(eg_bytes, eg[0], eg[1])
+    # find_bounds reads both key columns and writes a lower bound and a
+    # scanned count per left row.
+    res["find_bounds_roofline"] = roofline(4 * (n_r + n_s) + 8 * n_r, fb[0], fb[1])
     for b in list(src.values()) + list(work.values()) + [lb, lc, tot, out]:
         b.close()
     return res
@@ -614,106 +617,129 @@ def mem_info(rf):
     return int(f.value), int(t.value)
 
 
+def summary(name, res):
+    """One stderr line per op: its times and every roofline fraction in it
+    (the JSON line itself is too long for a log tail)."""
+    parts = []
+
+    def walk(prefix, d):
+        for k, v in d.items():
+            if not isinstance(v, dict):
+                if k in ("ms", "core_ms", "plan_ms", "batched_ms", "speedup", "speedup_qps"):
+                    parts.append(f"{prefix}{k}={v:.4g}")
+                continue
+            if k == "roofline" or k.endswith("_roofline") or k.endswith("_pass"):
+                tag = prefix + (k[: -len("_roofline")] if k.endswith("_roofline") else
+                                k if k != "roofline" else "")
+                tag = tag.rstrip(".")
+                parts.append(f"{tag or 'kernel'}: {v['launch_ms']:.4f} ms/launch "
+                             f"{v['achieved']:.0f} GB/s frac {v['frac']:.3f}")
+            elif k in ("saturation", "plain"):
+                parts.append(f"{prefix}{k} qps={v['qps']:.0f} p99={v['p99_ms']:.1f}ms")
+            elif k not in ("cpu_baseline", "autotune_timings_ms", "autotuned"):
+                walk(f"{prefix}{k}.", v)
+
+    walk("", res)
+    return f"  {name}: " + "; ".join(parts)
+
+
+def _host_cpu():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_line(run, rows, sample):
+    """A CPU baseline figure: `run(reps)` times the restated algorithm
+    (oracle/rfx_cpu.cpp) on the sample and returns the median seconds of
+    `reps` runs after one warm-up.  Timed on all the OpenMP threads the host
+    gives this process (OMP_NUM_THREADS), then on one thread."""
+    from oracle import cpu_baseline as cb
+
+    with cb.threads(0) as cores:
+        dt = run(5)
+    with cb.threads(1):
+        dt1 = run(3)
+    return {"value": round(rows / dt, 1), "unit": "rows/s", "cores": cores, "kind": "port",
+            "value_1_thread": round(rows / dt1, 1),
+            "sample": f"{sample}; median of 5 runs after a warm-up ({dt:.4f} s) on {cores} "
+                      f"threads, median of 3 ({dt1:.4f} s) on 1 thread; host {_host_cpu()}, "
+                      f"oracle/rfx_cpu.cpp (g++ -O3 -march=x86-64-v3 -fopenmp)"}
+
+
 def cpu_baseline_sort():
-    """The oracle's serial LSD sort (one thread) on 2^26 of the 2^28 pairs."""
-    import oracle
+    """The restated stable LSD radix sort (8-bit digits, as the reference's
+    b40c sort) on 2^25 of the 2^28 pairs."""
+    from oracle import cpu_baseline as cb
     from redfox_amd import datagen
 
-    n = 1 << 26
+    n = 1 << 25
     keys = datagen.u32(SEED + 1, n)
     vals = np.arange(n, dtype=np.uint32)
-    t0 = time.perf_counter()
-    oracle.sort_pairs(keys, vals, 4, 32, 4)
-    dt = time.perf_counter() - t0
-    return {"value": round(n / dt, 1), "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": f"2^26 (u32 key, u32 value) pairs of the same generator, serial LSD radix "
-                      f"sort in oracle/rfx_oracle.c, {dt:.2f} s on "
-                      f"{os.cpu_count()} visible host cores (1 used)"}
+    return cpu_line(lambda reps: cb.sort_pairs(keys, vals, reps)[0], n,
+                    "2^25 (u32 key, u32 value) pairs of the same generator, stable LSD radix "
+                    "sort, 4 passes of 8-bit digits")
 
 
-def cpu_ops_baseline():
-    """The oracle (serial C restatements of the reference algorithms, one
-    thread) on bounded samples of each op line's workload, as rows/s."""
-    import oracle
+def cpu_ops_baseline(names):
+    """The restated algorithms (OpenMP, all host threads and one) on bounded
+    samples of each op line's workload, as rows/s.  Lines whose work has no
+    restatement here (the text loader, the string join, the Q1 plan, the
+    batched and served forms) carry no CPU figure."""
+    from oracle import cpu_baseline as cb
     from redfox_amd import datagen
 
-    def timed(fn):
-        t0 = time.perf_counter()
-        fn()
-        return time.perf_counter() - t0
-
-    def line(rows, dt, sample):
-        return {"value": round(rows / dt, 1), "unit": "rows/s", "cores": 1, "kind": "port",
-                "sample": f"{sample}; {dt:.3f} s"}
-
     out = {}
-    n = 1 << 22
-    k = datagen.splitmix64(SEED + 20, n)
-    v = np.arange(n, dtype=np.uint32)
-    out["sort_u64_zipf_2^27"] = line(n, timed(lambda: oracle.sort_pairs(k, v, 8, 64, 4)),
-                                     "2^22 (u64 key, u32 value) pairs, serial LSD, 8 passes")
+    n = 1 << 24
+    if "sort_u64_zipf_2^27" in names:
+        k = datagen.mix64(datagen.zipf_ranks(SEED + 2, n, 1.1, 1 << 32))
+        v = np.arange(n, dtype=np.uint32)
+        out["sort_u64_zipf_2^27"] = cpu_line(
+            lambda reps: cb.sort_pairs(k, v, reps)[0], n,
+            "2^24 (u64 Zipf(1.1) key, u32 value) pairs, stable LSD radix sort, 8 passes")
     for name, nr, ns, kb in (("join_65536x262144", 65536, 262144, 16),
-                             ("join_2^24x2^26", 1 << 20, 1 << 22, 20)):
+                             ("join_2^24x2^26", 1 << 22, 1 << 24, 22)):
+        if name not in names:
+            continue
         rk, rv, sk, sv = datagen.join_tables(SEED, nr, ns, kb)
-        rk, rv = oracle.sort_pairs(rk, rv, 4, 32, 4)
-        sk, sv = oracle.sort_pairs(sk, sv, 4, 32, 4)
-
-        def core():
-            lb, lc, tot = oracle.join_find_bounds(rk, sk, 4)
-            li, ri = oracle.join_main(lb, lc, tot)
-            oracle.join_gather(16, rk, 4, rv, 4, sv, 4, li, ri, 64, 32)
-
+        o1, o2 = np.argsort(rk, kind="stable"), np.argsort(sk, kind="stable")
+        rk, rv, sk, sv = rk[o1], rv[o1], sk[o2], sv[o2]
         sample = (f"the same {nr} x {ns} join" if nr == 65536 else
-                  f"{nr} x {ns} rows, keys in [0, 2^{kb})") + ", find_bounds + expand + gather"
-        out[name] = line(nr + ns, timed(core), sample)
-    cols = datagen.lineitem(SEED + 2, 4_000_000)
-    out["q1_60M"] = line(4_000_000, timed(lambda: oracle.q1(*cols, 2436)), "4M lineitem rows")
-    mf_cols = [cols[0], cols[4], cols[5], cols[6]]
-    mf_out = [(("mul", ("col", 1), ("sub", ("int", 1), ("col", 2))), "f64"),
-              (("mul", ("mul", ("col", 1), ("sub", ("int", 1), ("col", 2))),
-                ("add", ("int", 1), ("col", 3))), "f64")]
-    out["mapfilter_2^27"] = line(
-        4_000_000, timed(lambda: oracle.mapfilter(("le", ("col", 0), ("int", 2436)), mf_out,
-                                                  mf_cols, 4_000_000)),
-        "4M lineitem rows, the same filter and two computed columns (numpy, vectorised)")
-    g = (datagen.splitmix64(SEED + 21, n) % np.uint64(1 << 16)).astype(np.uint32)
-    g.sort()
-    val = datagen.uniform01(SEED + 22, n) * 1e3
-    out["agg_2^28"] = line(n, timed(lambda: (oracle.reduce_by_key_f64(g, val, 4, 0),
-                                             oracle.reduce_by_key_u32(g, np.ones(n, np.uint32), 4))),
-                           "2^22 sorted rows, 2^16 groups, compensated SUM and COUNT")
-    from redfox_amd import loaders
-
-    tt = datagen.lineitem_tbl(1 << 14, SEED + 41)[0]
-    out["tbl_lineitem_4M"] = line(tt.count(b"\n"), timed(lambda: oracle.parse_tbl(tt, loaders.LINEITEM_SPEC)),
-                                  "about 2^16 dbgen-style lineitem rows, the oracle's parser "
-                                  "(Python, per field)")
-    skeys = [b"SUPPLIER#%09d" % i for i in range(1 << 12)]
-    srng = np.random.default_rng(SEED + 51)
-    sright = [skeys[i] for i in np.sort(srng.integers(0, 1 << 12, 1 << 16))]
-    out["join_string_2^18x2^22"] = line((1 << 12) + (1 << 16), timed(
-        lambda: oracle.find_bounds_string(skeys, sright)),
-        "2^12 left x 2^16 right string keys of the same form, the oracle's sorted search "
-        "(Python bisect)")
-    kl = datagen.lineitem_keyed(1 << 20, SEED + 31)
-
-    def q1_plan_np():
-        ship, rf_, ls_, qty, price, disc, tax = kl[2:]
-        keep = ship <= 912470400 - 90 * 86400
-        g = (rf_.astype(np.int64) * 2 + ls_)[keep]
-        dp = price * (1 - disc)
-        for w in (None, qty, price, disc, dp, dp * (1 + tax)):
-            np.bincount(g, weights=None if w is None else w[keep], minlength=6)
-    out["q1_plan_2^24"] = line(len(kl[0]), timed(q1_plan_np),
-                               "about 2^22 keyed lineitem rows: the plan's filter and its six "
-                               "GROUP BY results (numpy, vectorised)")
-    t = datagen.splitmix64(SEED + 23, 2 * n)
-    out["select_2^27"] = line(n, timed(lambda: oracle.select(t, 16, 64, 32, 2, 1 << 31)),
-                              "2^22 16-byte tuples, half kept")
-    a = np.unique(datagen.splitmix64(SEED + 24, n))
-    b = np.unique(datagen.splitmix64(SEED + 24 + 1, n))
-    out["setop_2^27"] = line(len(a) + len(b), timed(lambda: oracle.set_op(0, a, b, 8)),
-                             "union of two 2^22 sorted u64 sets")
+                  f"{nr} x {ns} rows, keys in [0, 2^{kb})")
+        out[name] = cpu_line(lambda reps: cb.join(rk, rv, sk, sv, reps)[0], nr + ns,
+                             sample + ", sorted inputs: sorted search, scan, expansion and "
+                             "gather of (key, r_val, s_val)")
+    if "q1_60M" in names or "mapfilter_2^27" in names:
+        cols = datagen.lineitem(SEED + 2, n)
+        if "q1_60M" in names:
+            out["q1_60M"] = cpu_line(lambda reps: cb.q1(*cols, 2436, reps)[0], n,
+                                     "2^24 lineitem rows of the same generator, Q1 filter and "
+                                     "6-group sums (Neumaier) and counts")
+        if "mapfilter_2^27" in names:
+            out["mapfilter_2^27"] = cpu_line(
+                lambda reps: cb.mapfilter(cols[0], cols[4], cols[5], cols[6], 2436, reps)[0], n,
+                "2^24 lineitem rows, the same filter and two computed columns")
+    if "agg_2^28" in names:
+        g = np.sort((datagen.mix64(datagen.zipf_ranks(SEED + 21, n, 1.1, 1 << 20)) %
+                     np.uint64(1 << 20)).astype(np.uint32))
+        val = datagen.uniform01(SEED + 22, n) * 1e3
+        out["agg_2^28"] = cpu_line(lambda reps: cb.reduce_sum_count(g, val, reps)[0], n,
+                                   "2^24 sorted rows, Zipf(1.1) keys over 2^20 values, "
+                                   "SUM (Neumaier) and COUNT per run")
+    if "select_2^27" in names:
+        t = datagen.splitmix64(SEED + 23, 2 * n)
+        out["select_2^27"] = cpu_line(lambda reps: cb.select16(t, 1 << 31, reps)[0], n,
+                                      "2^24 16-byte tuples, half kept, stable compaction")
+    if "setop_2^27" in names:
+        a = np.unique(datagen.splitmix64(SEED + 24, n))
+        b = np.unique(np.concatenate([a[len(a) // 2:], datagen.splitmix64(SEED + 25, n // 2)]))
+        out["setop_2^27"] = cpu_line(lambda reps: cb.set_op(0, a, b, reps)[0], len(a) + len(b),
+                                     "union of two 2^24 sorted u64 sets, half of B in A")
     return out
 
 
@@ -769,14 +795,14 @@ def main():
             if only is None or name in only:
                 log(what + " ...")
                 ops[name] = fn()
+                log(summary(name, ops[name]))
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
         cpu = cpu_baseline_sort()
         if not args.no_ops:
-            for name, base in cpu_ops_baseline().items():
-                if name in ops:
-                    ops[name]["cpu_baseline"] = base
+            for name, base in cpu_ops_baseline(set(ops)).items():
+                ops[name]["cpu_baseline"] = base
     line = {
         "metric": METRIC,
         "value": round(s["rows_per_s"], 1),
