@@ -57,6 +57,7 @@ extern "C" {
 #define RFX_I128 5
 #define RFX_F32 9
 #define RFX_F64 10
+#define RFX_POINTER 11  /* a string: the address of NUL-terminated bytes */
 
 typedef struct rfx_ctx rfx_ctx;
 
@@ -268,9 +269,10 @@ int rfx_join_expand_project(rfx_ctx* ctx, const int32_t* lower_bound,
  * Reduce.cu:175-1532).  keys/vals hold *n rows (device u64 *n);
  * out_keys/out_vals receive one row per run and *n_out (device u64) the
  * run count.  op: RFX_AGG_SUM, RFX_AGG_MIN, RFX_AGG_MAX.  Outputs must
- * not overlap keys or vals (as for thrust::reduce_by_key): an output whose
- * first min(n_cap, out_cap) rows overlap the inputs' first n_cap rows
- * gives RFX_EINVAL. */
+ * not overlap keys or vals (as for thrust::reduce_by_key).  The inputs are
+ * taken to hold n_cap rows; an output that overlaps them gives RFX_EINVAL,
+ * where an output spans min(n_cap, out_cap) rows when out_cap is given
+ * and its first row otherwise.  Any other overlap is undefined. */
 #define RFX_AGG_SUM 0
 #define RFX_AGG_MIN 1
 #define RFX_AGG_MAX 2
@@ -550,6 +552,36 @@ void set_unique(void* begin, uint64_t* bytes, uint64_t type);
  * RFX_OK, or RFX_EINVAL for an unknown family / wrong argument count. */
 int rfx_harmony_call(const char* kernel_name, const uint64_t* args,
                      uint32_t nargs);
+
+/* One Harmony kernel record run on the native operators: what
+ * PTXKernelLaunch::execute (Runtime.cpp:506-646, kernel_type 1) and
+ * BinaryKernelLaunch::execute (:672-1038, kernel_type 2) do, with the
+ * operand slots packed as they pack them: for kernel_type 1, args[0..2]
+ * are the values of the ctaCount, threadCount and sharedBytes variables
+ * and every later slot a device address; for kernel_type 2, the family's
+ * pointer slots are addresses and the rest the first 8 bytes of their
+ * variables.  `code` is the HIR kernel's code bytes: an rfx1 descriptor
+ * holding what RelationalAlgebraKernel::cudaSourceRepresentation()
+ * (RelationalAlgebraKernel.cpp:849-909) bakes into the instance:
+ *   rfx1 <family> [d=T] [d1=T] [a=T] [b=T] [c=T] [e=T] [dom=N]
+ *        [map=i,j,...] [agg=total|min|max|count] [cmp=FIELD:OP:CONST]
+ *        [parts=N] [nt=N]
+ * T lists a tuple's field widths in bits, each optionally followed by f
+ * (float) or p (a string address), or "-" for none.  Compute families:
+ * split_get_result_size, split_key_get_result_size, set_split,
+ * set_splitkey, merge_get_result_size, set_merge,
+ * generate_get_result_size, set_generate, join_get_temp_size,
+ * join_get_result_size, mgpu_join_gather, union_get_result_size,
+ * difference_get_result_size, project_get_result_size, set_project,
+ * set_select (field-vs-constant), scan, select_get_result_size,
+ * select_gather.  Binary families: as rfx_harmony_call but with the
+ * reference's operand count, plus string (Pointer) keys in
+ * set_mgpusortpair/key, mgpu_join_find_bounds, set_reduce and set_count.
+ * Runs on the default context and finishes before returning; failures
+ * return RFX_E* (RFX_ENOTSUP for a family that is not provided). */
+int rfx_harmony_launch(const char* kernel_name, int kernel_type,
+                       const void* code, uint64_t code_bytes,
+                       const uint64_t* args, uint32_t nargs);
 
 /* ---------------------------------------------------------------------- */
 /* MAPFILTER: fused filter / project / arithmetic over columns            */

@@ -109,6 +109,7 @@ SIGNATURES = {
     "set_intersection": (None, [P, P, P, U64, P, U64, U64]),
     "set_unique": (None, [P, P, U64]),
     "rfx_harmony_call": (I, [c_char_p, POINTER(c_uint64), U32]),
+    "rfx_harmony_launch": (I, [c_char_p, I, c_char_p, U64, POINTER(c_uint64), U32]),
     "rfx_mapfilter": (I, [P, P, I, U64, P, P, I, P]),
     "rfx_tbl_parse": (I, [P, P, U64, ctypes.c_char, P, I, P, U64, P, P]),
     "rfx_join_find_bounds_string": (I, [P, P, P, P, P, U64, P, U64]),

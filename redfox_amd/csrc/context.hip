@@ -105,6 +105,21 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
     p[i] = v;
 }
 
+int ctx_cells(rfx_ctx* ctx, uint64_t** cells) {
+  if (!ctx->cells) {
+    if (ctx->capturing) return fail(RFX_EINVAL, "context cells must be allocated before capture");
+    RFX_HIP(hipSetDevice(ctx->device));
+    void* p = nullptr;
+    if (hipMalloc(&p, kCtxCells * sizeof(uint64_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(RFX_ENOMEM, "allocating the context's scalar cells failed");
+    }
+    ctx->cells = static_cast<uint64_t*>(p);
+  }
+  *cells = ctx->cells;
+  return RFX_OK;
+}
+
 int fill_u32(rfx_ctx* ctx, uint32_t* p, uint64_t words, uint32_t value) {
   if (words == 0) return RFX_OK;
   uint64_t blocks = (words + 255) / 256;
@@ -256,6 +271,10 @@ int rfx_ctx_destroy(rfx_ctx* ctx) {
   if (ctx->ws && !ctx->ws_borrowed) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(ctx->ws);
+  }
+  if (ctx->cells) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->cells);
   }
   delete ctx;
   return RFX_OK;

@@ -11,7 +11,6 @@
 // the default context, finish before returning, and abort on failure as the
 // reference's check() does.  rfx_harmony_call() is the packed dispatcher.
 #include <cstring>
-#include <mutex>
 
 #include "rfx_device.h"
 #include "rfx_internal.h"
@@ -34,15 +33,12 @@ void finish(const char* where, rfx_ctx* c, int status) {
   if (status != RFX_OK) die(where, status);
 }
 
-// A few device u64 cells for scalar operands the native API reads on the
-// device.
-uint64_t* cells(const char* where) {
-  static std::once_flag once;
-  static uint64_t* p = nullptr;
-  std::call_once(once, [] {
-    if (hipMalloc(&p, 8 * sizeof(uint64_t)) != hipSuccess) p = nullptr;
-  });
-  if (!p) die(where, RFX_ENOMEM);
+// The default context's device u64 cells, for scalar operands the native
+// API reads on the device.
+uint64_t* cells(const char* where, rfx_ctx* c) {
+  uint64_t* p = nullptr;
+  const int st = ctx_cells(c, &p);
+  if (st != RFX_OK) die(where, st);
   return p;
 }
 
@@ -118,7 +114,7 @@ void mgpu_join_find_bounds(int32_t* lower_bound, int32_t* left_count, uint64_t* 
 void mgpu_join_main(int32_t* left_idx, int32_t* right_idx, uint64_t idx_bytes,
                     int32_t* lower_bound, int32_t* left_count, uint64_t lb_bytes) {
   rfx_ctx* c = hctx("mgpu_join_main");
-  uint64_t* cell = cells("mgpu_join_main");
+  uint64_t* cell = cells("mgpu_join_main", c);
   const uint64_t total = idx_bytes / 4;
   write_cell("mgpu_join_main", cell, total);
   finish("mgpu_join_main", c,
@@ -139,7 +135,7 @@ void mgpu_join_gather(void* result, void* left_key, int32_t* left_index, void* r
   const int ob = desc & 0xFF, kb = (desc >> 8) & 0xFF, lvb = (desc >> 16) & 0xFF,
             rvb = (desc >> 24) & 0xFF;
   const uint32_t kshift = (desc >> 32) & 0xFF, lvshift = (desc >> 40) & 0xFF;
-  uint64_t* cell = cells("mgpu_join_gather");
+  uint64_t* cell = cells("mgpu_join_gather", c);
   const uint64_t total = read_cell("mgpu_join_gather", size) / 4;
   write_cell("mgpu_join_gather", cell + 1, total);
   finish("mgpu_join_gather", c,

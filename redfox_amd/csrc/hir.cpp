@@ -28,17 +28,23 @@
 //   (:1271-1290), Free releases the memory (:1298-1314);
 // * a binary kernel's operand i is passed as the variable's device address
 //   when i is in its family's pointer set, else as the first 8 bytes of the
-//   variable (BinaryKernelLaunch::execute :672-1038, getVariable :99-118).
-//   Operands past the reference family's count (the per-instance type
-//   descriptor a compiler adds for rfx) are values.
+//   variable (BinaryKernelLaunch::execute :672-1038, getVariable :99-118);
+// * a compute (PTX) kernel's operands 0-2 are the values of its ctaCount,
+//   threadCount and sharedBytes variables and the rest device addresses
+//   (PTXKernelLaunch::execute :506-646);
+// * both run through rfx_harmony_launch(), which takes what the generated
+//   source baked into the instance (tuple widths, key fields, ...) from an
+//   rfx1 descriptor in the kernel's `code` bytes (harmony_desc.h).  A binary
+//   kernel without a descriptor goes to rfx_harmony_call(), whose families
+//   take those types as one operand past the reference's count.
 //
 // Differences, each where Harmony's behaviour is undefined or missing:
 // a freed variable's address is cleared (Harmony keeps the dangling
 // pointer); set_intersection uses set_union's pointer set (Harmony has no
 // branch for it and passes uninitialised slots); the self-test records,
 // which Harmony loads but never runs (HirManager.cpp:76-163, compiled
-// out), are run and compared byte for byte.  PTX compute kernels cannot
-// run on gfx950 and are rejected with RFX_ENOTSUP.
+// out), are run and compared byte for byte.  Variables take their device
+// memory from a pool kept for the life of the program.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,6 +56,7 @@
 #include <string>
 #include <vector>
 
+#include "harmony_desc.h"
 #include "rfx_internal.h"
 
 namespace rfx {
@@ -130,7 +137,8 @@ struct Operand {
 
 struct Kernel {
   std::string name;
-  uint64_t code_bytes = 0;  // the code itself is not kept
+  uint64_t code_bytes = 0;
+  std::string desc;  // the code when it is an rfx1 descriptor; other code is not kept
   int type = 0;
   std::vector<Operand> ops;
   std::vector<uint32_t> targets;
@@ -199,6 +207,8 @@ bool parse_kernel(Wire w, Kernel* k) {
     else if (f == 2 && wt == 2) {
       const Wire c = w.sub();
       k->code_bytes = static_cast<uint64_t>(c.end - c.p);
+      if (is_desc(reinterpret_cast<const char*>(c.p), k->code_bytes))
+        k->desc.assign(reinterpret_cast<const char*>(c.p), k->code_bytes);
       seen |= 2;
     }
     else if (f == 3 && wt == 0) { k->type = static_cast<int>(w.varint()); seen |= 4; }
@@ -292,8 +302,47 @@ const Family* family_of(const std::string& name) {
   return nullptr;
 }
 
+// Device memory of the variables, pooled by power-of-two capacity (256
+// bytes and up) for the life of the program: a variable freed by liveness
+// returns its block to the pool and the next allocation of that class
+// reuses it.  Every kernel finishes before the next operation runs, so a
+// reused block is never still in use.
+struct Pool {
+  std::map<uint64_t, std::vector<void*>> free;
+  std::vector<void*> all;
+  static uint64_t cls(uint64_t bytes) {
+    uint64_t c = 256;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  int take(uint64_t bytes, void** p, uint64_t* cap) {
+    const uint64_t c = cls(bytes);
+    auto& v = free[c];
+    if (!v.empty()) {
+      *p = v.back();
+      v.pop_back();
+    } else {
+      if (hipMalloc(p, c) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(RFX_ENOMEM, "hir: allocating %llu bytes", static_cast<unsigned long long>(c));
+      }
+      all.push_back(*p);
+    }
+    *cap = c;
+    return RFX_OK;
+  }
+  void give(void* p, uint64_t cap) { free[cap].push_back(p); }
+  void release() {
+    for (void* p : all) (void)hipFree(p);
+    all.clear();
+    free.clear();
+  }
+};
+
 struct VarState {
   void* addr = nullptr;
+  uint64_t block = 0;      // pool class of addr
   uint64_t size_word = 0;  // Harmony's sizeInElements
   uint64_t bytes = 0;      // current logical size in bytes
   uint64_t capacity = 0;   // bytes allocated
@@ -321,6 +370,7 @@ struct rfx_hir {
   std::vector<std::vector<rfx::Op>> schedule;
   std::map<uint32_t, rfx::VarState> state;
   std::map<uint32_t, std::string> overrides;  // input contents set by the caller or a test
+  rfx::Pool pool;
 };
 
 namespace rfx {
@@ -468,20 +518,28 @@ void schedule(rfx_hir* h) {
   }
 }
 
-void release_all(rfx_hir* h) {
+// Back to the pool (between runs) or to the device (at destroy).
+void release_all(rfx_hir* h, bool destroy = false) {
   for (auto& kv : h->state)
-    if (kv.second.addr) (void)hipFree(kv.second.addr);
+    if (kv.second.addr) h->pool.give(kv.second.addr, kv.second.block);
   h->state.clear();
+  if (destroy) h->pool.release();
+}
+
+void free_var(rfx_hir* h, uint32_t name) {
+  VarState& s = h->state[name];
+  if (s.addr) h->pool.give(s.addr, s.block);
+  s = VarState{};
 }
 
 int alloc_var(rfx_hir* h, uint32_t name, uint64_t bytes) {
   VarState& s = h->state[name];
   if (s.addr) return fail(RFX_EINVAL, "hir: variable %u allocated twice", name);
   void* p = nullptr;
-  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess)
-    return fail(RFX_ENOMEM, "hir: allocating %llu bytes for variable %u",
-                static_cast<unsigned long long>(bytes), name);
+  uint64_t cls = 0;
+  RFX_TRY(h->pool.take(bytes ? bytes : 1, &p, &cls));
   s.addr = p;
+  s.block = cls;
   s.capacity = bytes;
   return RFX_OK;
 }
@@ -492,6 +550,16 @@ int value_of(rfx_hir* h, uint32_t name, uint64_t* v) {
   if (!s.addr) return fail(RFX_EINVAL, "hir: reading unallocated variable %u", name);
   *v = 0;
   RFX_HIP(hipMemcpy(v, s.addr, std::min<uint64_t>(8, s.capacity), hipMemcpyDeviceToHost));
+  return RFX_OK;
+}
+
+int address_of(rfx_hir* h, const Kernel& k, size_t i, uint64_t* slot) {
+  const uint32_t v = k.ops[i].var;
+  const VarState& s = h->state[v];
+  if (!s.addr)
+    return fail(RFX_EINVAL, "hir: kernel '%s' operand %zu: variable %u is not allocated",
+                k.name.c_str(), i, v);
+  *slot = reinterpret_cast<uint64_t>(s.addr);
   return RFX_OK;
 }
 
@@ -531,18 +599,27 @@ int run_kernel(rfx_hir* h, const Kernel& k) {
       if (!fam) return fail(RFX_ENOTSUP, "hir: no kernel family for binary kernel '%s'", k.name.c_str());
       std::vector<uint64_t> slots(k.ops.size());
       for (size_t i = 0; i < k.ops.size(); ++i) {
-        const uint32_t v = k.ops[i].var;
-        if (i < 32 && ((fam->pointers >> i) & 1)) {
-          const VarState& s = h->state[v];
-          if (!s.addr)
-            return fail(RFX_EINVAL, "hir: kernel '%s' operand %zu: variable %u is not allocated",
-                        k.name.c_str(), i, v);
-          slots[i] = reinterpret_cast<uint64_t>(s.addr);
-        } else {
-          RFX_TRY(value_of(h, v, &slots[i]));
-        }
+        if (i < 32 && ((fam->pointers >> i) & 1)) RFX_TRY(address_of(h, k, i, &slots[i]));
+        else RFX_TRY(value_of(h, k.ops[i].var, &slots[i]));
       }
+      // With a descriptor the operands are exactly the reference's; without
+      // one the compatibility form takes the instance's types as a trailing
+      // operand.
+      if (!k.desc.empty())
+        return rfx_harmony_launch(k.name.c_str(), kBinary, k.desc.data(), k.desc.size(),
+                                  slots.data(), static_cast<uint32_t>(slots.size()));
       return rfx_harmony_call(k.name.c_str(), slots.data(), static_cast<uint32_t>(slots.size()));
+    }
+    case kCompute: {
+      // Launch dimensions by value, every other operand by address
+      // (PTXKernelLaunch::execute).
+      std::vector<uint64_t> slots(k.ops.size());
+      for (size_t i = 0; i < k.ops.size(); ++i) {
+        if (i < 3) RFX_TRY(value_of(h, k.ops[i].var, &slots[i]));
+        else RFX_TRY(address_of(h, k, i, &slots[i]));
+      }
+      return rfx_harmony_launch(k.name.c_str(), kCompute, k.desc.data(), k.desc.size(), slots.data(),
+                                static_cast<uint32_t>(slots.size()));
     }
     case kGetSize: {
       VarState& t = h->state[k.ops[0].var];
@@ -573,8 +650,6 @@ int run_kernel(rfx_hir* h, const Kernel& k) {
       s.bytes = std::min(value, s.capacity);
       return RFX_OK;
     }
-    case kCompute:
-      return fail(RFX_ENOTSUP, "hir: PTX kernel '%s' cannot run on gfx950", k.name.c_str());
     default:
       return fail(RFX_ENOTSUP, "hir: kernel '%s' of type %d inside a block", k.name.c_str(), k.type);
   }
@@ -591,9 +666,7 @@ int run_program(rfx_hir* h) {
         VarState& s = h->state[op.var];
         s.size_word = s.bytes = size;
       } else if (op.kind == kFree) {
-        VarState& s = h->state[op.var];
-        if (s.addr) (void)hipFree(s.addr);
-        s = VarState{};
+        free_var(h, op.var);
       } else {
         RFX_TRY(run_kernel(h, *op.kernel));
       }
@@ -636,7 +709,8 @@ int rfx_hir_load(rfx_hir** out, const void* bytes, uint64_t nbytes) {
 
 void rfx_hir_destroy(rfx_hir* h) {
   if (!h) return;
-  release_all(h);
+  (void)hipDeviceSynchronize();
+  release_all(h, true);
   delete h;
 }
 

@@ -66,7 +66,7 @@ __device__ __forceinline__ uint64_t merge_path_lower(const K* a, uint64_t na, co
 #endif
 // find_bounds tile: merged (left + right) elements per block.
 constexpr int kFbNV = RFX_FB_NV;
-constexpr int kFbNT = 512;
+constexpr int kFbNT = 256;
 
 template <class K>
 __global__ __launch_bounds__(256) void k_fb_partition(const K* __restrict__ a, uint64_t na,
@@ -80,14 +80,23 @@ __global__ __launch_bounds__(256) void k_fb_partition(const K* __restrict__ a, u
   split[t] = merge_path_lower(a, na, b, nb, diag);
 }
 
+// LDS index of count i: one pad word per 32, so that a thread reading its
+// VT consecutive counts and lanes writing thread-strided counts both stay
+// (nearly) free of bank conflicts.
+__device__ __forceinline__ uint32_t cnt_slot(uint32_t i) { return i + (i >> 5); }
+
 // One tile of the merge path: its left keys A[a0, a1) and right keys
 // B[b0, b1).  With A winning ties, every B[j] before b0 is < A[a0] and every
 // B[j] from b1 on is >= A[a1 - 1], so each left key's lower bound lies in
-// [b0, b1] and is found by a binary search of the tile's right keys in LDS
-// (one left key per lane: neighbouring lanes probe the same words), and so
-// is its upper bound; the equal-key count is their difference, extended by
-// a gallop through global memory only for a run that crosses the tile's
-// end.
+// [b0, b1].  The tile's right keys are followed in LDS by the next NT right
+// keys (B[b1, b1 + NT)), loaded with the rest, so the equal run of a left
+// key that crosses the tile's end - the last key of nearly every tile - is
+// measured in LDS too; only a run longer than that continues in global
+// memory (a gallop from b1 + NT).  Lower and upper bounds come from
+// fixed-step binary searches in LDS, four left keys per thread in lockstep
+// (neighbouring lanes probe the same words).  The counts are then scanned
+// in left-key order across the block and, by decoupled look-back, across
+// tiles.
 template <class K, int NT, int NV>
 __global__ __launch_bounds__(NT) void k_find_bounds(
     const K* __restrict__ a, uint64_t na, const K* __restrict__ b, uint64_t nb,
@@ -95,11 +104,17 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
     int32_t* __restrict__ left_count, uint64_t* __restrict__ total,
     uint64_t* __restrict__ status, uint32_t* __restrict__ ticket) {
   constexpr int VT = NV / NT;
-  __shared__ K keys[NV];
-  __shared__ uint32_t s_cnt[NV];
+  static_assert(VT % 4 == 0, "left keys are searched four per thread at a time");
+  constexpr int EXT = NT;
+  constexpr int KEY_BYTES = (NV + EXT) * static_cast<int>(sizeof(K));
+  constexpr int CNT_BYTES = (NV + NV / 32) * 4;
+  // The keys are dead once the searches are done; the counts reuse them.
+  __shared__ __attribute__((aligned(16))) unsigned char smem[KEY_BYTES > CNT_BYTES ? KEY_BYTES : CNT_BYTES];
   __shared__ uint64_t scan_tmp[NT / kWave + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
+  K* keys = reinterpret_cast<K*>(smem);
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(smem);
 
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
@@ -110,83 +125,94 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
   const uint64_t a0 = split[tile], a1 = split[tile + 1];
   const uint64_t b0 = d0 - a0, b1 = d1 - a1;
   const uint32_t nat = static_cast<uint32_t>(a1 - a0), nbt = static_cast<uint32_t>(b1 - b0);
+  const uint32_t next = static_cast<uint32_t>(nb - b1 < EXT ? nb - b1 : EXT);
 
-  // Tile's A keys then B keys, contiguous in LDS; all loads in flight
-  // before the first LDS write.
-  if constexpr (sizeof(K) <= 8) {
+  // The tile's A keys, its B keys, then the B keys past its end: every
+  // load in flight before the first LDS write.
+  {
     K v[VT];
+    K ev{};
 #pragma unroll
     for (int j = 0; j < VT; ++j) {
       const uint32_t i = tid + j * NT;
       if (i < nat) v[j] = a[a0 + i];
       else if (i < nat + nbt) v[j] = b[b0 + (i - nat)];
     }
+    if (tid < next) ev = b[b1 + tid];
 #pragma unroll
     for (int j = 0; j < VT; ++j) {
       const uint32_t i = tid + j * NT;
       if (i < nat + nbt) keys[i] = v[j];
     }
-  } else {
-    for (uint32_t i = tid; i < nat + nbt; i += NT)
-      keys[i] = i < nat ? a[a0 + i] : b[b0 + (i - nat)];
+    if (tid < next) keys[nat + nbt + tid] = ev;
   }
   __syncthreads();
   const K* sa = keys;
   const K* sb = keys + nat;
+  const uint32_t nbx = nbt + next;  // right keys in LDS
 
-  // Lower and upper bounds of each left key in the tile's right keys by
-  // fixed-step binary searches: all of a thread's searches advance in
-  // lockstep, so their LDS reads overlap.  JA (block-uniform) bounds the
-  // left keys per thread.
+  // Left key i = tid + j * NT; JA (block-uniform) bounds j.
   const uint32_t JA = (nat + NT - 1) / NT;
-  uint32_t top = 0;
-  if (nbt) top = 1u << (31 - __builtin_clz(nbt));
-  K kv[VT];
-  uint32_t lo[VT], up[VT];
+  const uint32_t top_lo = nbt ? 1u << (31 - __builtin_clz(nbt)) : 0u;
+  const uint32_t top_up = nbx ? 1u << (31 - __builtin_clz(nbx)) : 0u;
+  uint32_t cnt[VT];
 #pragma unroll
-  for (int j = 0; j < VT; ++j) {
-    lo[j] = up[j] = 0;
-    if (static_cast<uint32_t>(j) < JA) {
-      const uint32_t i = tid + j * NT;
-      kv[j] = sa[i < nat ? i : nat - 1];
-    }
-  }
-  for (uint32_t step = top; step; step >>= 1) {
+  for (int j = 0; j < VT; ++j) cnt[j] = 0;
 #pragma unroll
-    for (int j = 0; j < VT; ++j) {
-      if (static_cast<uint32_t>(j) < JA) {
-        if (lo[j] + step <= nbt && lt(sb[lo[j] + step - 1], kv[j])) lo[j] += step;
-        if (up[j] + step <= nbt && !lt(kv[j], sb[up[j] + step - 1])) up[j] += step;
+  for (int jb = 0; jb < VT; jb += 4) {
+    if (static_cast<uint32_t>(jb) < JA) {
+      K kv[4];
+      uint32_t lo[4], up[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t i = tid + (jb + q) * NT;
+        kv[q] = sa[i < nat ? i : nat - 1];
+        lo[q] = up[q] = 0;
+      }
+      for (uint32_t step = top_lo; step; step >>= 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (lo[q] + step <= nbt && lt(sb[lo[q] + step - 1], kv[q])) lo[q] += step;
+      }
+      for (uint32_t step = top_up; step; step >>= 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (up[q] + step <= nbx && !lt(kv[q], sb[up[q] + step - 1])) up[q] += step;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t i = tid + (jb + q) * NT;
+        if (i < nat) {
+          uint64_t c = up[q] - lo[q];
+          // A run reaching past the keys in LDS continues in global memory.
+          const uint64_t bx = b1 + next;
+          if (up[q] == nbx && bx < nb && eq(b[bx], kv[q])) {
+            uint64_t l = bx + 1, st = 1, h;
+            while (true) {  // invariant: b[l - 1] == key
+              h = l + st - 1;
+              if (h >= nb) { h = nb; break; }
+              if (!eq(b[h], kv[q])) break;
+              l = h + 1;
+              st <<= 1;
+            }
+            while (l < h) {  // first index in [l, h) whose key differs
+              const uint64_t mid = (l + h) >> 1;
+              if (eq(b[mid], kv[q])) l = mid + 1;
+              else h = mid;
+            }
+            c += l - bx;
+          }
+          lower_bound[a0 + i] = static_cast<int32_t>(b0 + lo[q]);
+          cnt[jb + q] = static_cast<uint32_t>(c);
+        }
       }
     }
   }
+  __syncthreads();  // every search is done with the keys
 #pragma unroll
   for (int j = 0; j < VT; ++j) {
     const uint32_t i = tid + j * NT;
-    if (static_cast<uint32_t>(j) < JA && i < nat) {
-      const K key = kv[j];
-      uint64_t cnt = up[j] - lo[j];
-      // A run of equal right keys that reaches the tile's end continues
-      // in global memory from b1.
-      if (up[j] == nbt && b1 < nb && eq(b[b1], key)) {
-        uint64_t l = b1 + 1, step = 1, h;
-        while (true) {  // invariant: b[l - 1] == key
-          h = l + step - 1;
-          if (h >= nb) { h = nb; break; }
-          if (!eq(b[h], key)) break;
-          l = h + 1;
-          step <<= 1;
-        }
-        while (l < h) {  // first index in [l, h) whose key differs
-          const uint64_t mid = (l + h) >> 1;
-          if (eq(b[mid], key)) l = mid + 1;
-          else h = mid;
-        }
-        cnt += l - b1;
-      }
-      lower_bound[a0 + i] = static_cast<int32_t>(b0 + lo[j]);
-      s_cnt[i] = static_cast<uint32_t>(cnt);
-    }
+    if (i < nat) s_cnt[cnt_slot(i)] = cnt[j];
   }
   __syncthreads();
 
@@ -197,7 +223,7 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
   uint64_t my_sum = 0;
 #pragma unroll
   for (int j = 0; j < VT; ++j) {
-    cv[j] = (c0 + j < nat) ? s_cnt[c0 + j] : 0u;
+    cv[j] = (c0 + j < nat) ? s_cnt[cnt_slot(c0 + j)] : 0u;
     my_sum += cv[j];
   }
   uint64_t tile_sum;
@@ -219,13 +245,13 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
   }
 #pragma unroll
   for (int j = 0; j < VT; ++j) {
-    if (c0 + j < nat) s_cnt[c0 + j] = static_cast<uint32_t>(run);  // tile-local prefix
+    if (c0 + j < nat) s_cnt[cnt_slot(c0 + j)] = static_cast<uint32_t>(run);  // tile-local prefix
     run += cv[j];
   }
   __syncthreads();
   const uint64_t prefix = s_prefix;
   for (uint32_t i = tid; i < nat; i += NT)
-    left_count[a0 + i] = static_cast<int32_t>(prefix + s_cnt[i]);
+    left_count[a0 + i] = static_cast<int32_t>(prefix + s_cnt[cnt_slot(i)]);
   if (tile == ntiles - 1 && tid == 0) *total = prefix + tile_sum;
 }
 
@@ -277,10 +303,31 @@ struct GatherSpec {
   uint32_t lval_shift;
 };
 
-__device__ __forceinline__ u128 compose(const GatherSpec& g, uint64_t li, uint64_t ri) {
-  u128 o = shl128(load_w(g.lkey, li, g.lkey_bytes), g.key_shift);
-  if (g.lval) o = or128(o, shl128(load_w(g.lval, li, g.lval_bytes), g.lval_shift));
-  if (g.rval) o = or128(o, load_w(g.rval, ri, g.rval_bytes));
+// A source element held in S: uint64_t when every source is at most 8
+// bytes wide, u128 otherwise (fewer registers for the common case).
+template <class S>
+__device__ __forceinline__ S load_src(const void* base, uint64_t i, int w);
+template <>
+__device__ __forceinline__ uint64_t load_src<uint64_t>(const void* base, uint64_t i, int w) {
+  switch (w) {
+    case 1: return static_cast<const uint8_t*>(base)[i];
+    case 2: return static_cast<const uint16_t*>(base)[i];
+    case 4: return static_cast<const uint32_t*>(base)[i];
+    default: return static_cast<const uint64_t*>(base)[i];
+  }
+}
+template <>
+__device__ __forceinline__ u128 load_src<u128>(const void* base, uint64_t i, int w) {
+  return load_w(base, i, w);
+}
+
+// The output tuple of one joined row from its sources, already loaded:
+// (key << key_shift) | (left value << lval_shift) | right value.
+template <class S>
+__device__ __forceinline__ u128 compose_v(const GatherSpec& g, S k, S lv, S rv) {
+  u128 o = shl128(to128(k), g.key_shift);
+  if (g.lval) o = or128(o, shl128(to128(lv), g.lval_shift));
+  if (g.rval) o = or128(o, to128(rv));
   return mask128(o, 8u * static_cast<uint32_t>(g.out_bytes));
 }
 
@@ -300,11 +347,12 @@ struct EmitSpec {
   EmitField f[8];
 };
 
-__device__ __forceinline__ void emit(const EmitSpec& e, uint64_t g, uint64_t li, uint64_t ri) {
+// The projected output row of sources v0 (left key), v1 (left value) and
+// v2 (right value), already loaded.
+template <class S>
+__device__ __forceinline__ void emit_v(const EmitSpec& e, uint64_t g, S s0, S s1, S s2) {
   const u128 zero{0, 0};
-  const u128 v0 = (e.used & 1) ? load_w(e.src[0], li, e.src_bytes[0]) : zero;
-  const u128 v1 = (e.used & 2) ? load_w(e.src[1], li, e.src_bytes[1]) : zero;
-  const u128 v2 = (e.used & 4) ? load_w(e.src[2], ri, e.src_bytes[2]) : zero;
+  const u128 v0 = to128(s0), v1 = to128(s1), v2 = to128(s2);
   u128 o0 = zero, o1 = zero;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -333,8 +381,12 @@ constexpr int kExNT = 512;
 // <= g}, found by fixed-step binary searches of the slots' starts that a
 // thread runs in lockstep for all its outputs; its right index is the
 // row's lower bound plus g - start.  Outputs are thread-strided, so every
-// store is coalesced.
-template <int MODE, int NT, int NV>
+// store is coalesced.  The gathering modes load the sources of kExBatch
+// outputs before storing any of them: a store may alias a later load as
+// far as the compiler knows, so loads interleaved with stores would wait
+// for each other one memory latency at a time.
+constexpr int kExBatch = 4;
+template <int MODE, int NT, int NV, class S>
 __global__ __launch_bounds__(NT) void k_expand(
     const int32_t* __restrict__ lower_bound, const int32_t* __restrict__ starts, uint64_t nrows,
     const uint64_t* __restrict__ total_p, uint64_t total_cap, const uint64_t* __restrict__ split,
@@ -375,40 +427,107 @@ __global__ __launch_bounds__(NT) void k_expand(
       }
     }
   }
+  static_assert(VT % kExBatch == 0, "outputs are gathered in batches");
 #pragma unroll
-  for (int j = 0; j < VT; ++j) {
-    const uint32_t i = tid + j * NT;
-    if (static_cast<uint32_t>(j) < JA && i < ng) {
-      const uint32_t g = static_cast<uint32_t>(g0) + i;
-      const uint32_t sl = slot[j];
-      const uint32_t row = static_cast<uint32_t>(r0 - 1 + sl);
-      const uint32_t rix = s_lb[sl] + (g - s_start[sl]);
-      if constexpr (MODE == 0) {
-        left_idx[g] = static_cast<int32_t>(row);
-        right_idx[g] = static_cast<int32_t>(rix);
-      } else if constexpr (MODE == 2) {
-        emit(es, g, row, rix);
-      } else {
-        store_w(gs.out, g, gs.out_bytes, compose(gs, row, rix));
+  for (int jb = 0; jb < VT; jb += kExBatch) {
+    if (static_cast<uint32_t>(jb) >= JA) break;
+    uint32_t gq[kExBatch], rowq[kExBatch], rixq[kExBatch];
+    bool okq[kExBatch];
+#pragma unroll
+    for (int q = 0; q < kExBatch; ++q) {
+      const uint32_t i = tid + (jb + q) * NT;
+      okq[q] = static_cast<uint32_t>(jb + q) < JA && i < ng;
+      const uint32_t sl = slot[jb + q];
+      gq[q] = static_cast<uint32_t>(g0) + i;
+      rowq[q] = static_cast<uint32_t>(r0 - 1 + sl);
+      rixq[q] = okq[q] ? s_lb[sl] + (gq[q] - s_start[sl]) : 0u;
+    }
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int q = 0; q < kExBatch; ++q)
+        if (okq[q]) {
+          left_idx[gq[q]] = static_cast<int32_t>(rowq[q]);
+          right_idx[gq[q]] = static_cast<int32_t>(rixq[q]);
+        }
+    } else {
+      S x0[kExBatch], x1[kExBatch], x2[kExBatch];
+      const void* p0 = MODE == 1 ? gs.lkey : es.src[0];
+      const void* p1 = MODE == 1 ? gs.lval : es.src[1];
+      const void* p2 = MODE == 1 ? gs.rval : es.src[2];
+      const int w0 = MODE == 1 ? gs.lkey_bytes : es.src_bytes[0];
+      const int w1 = MODE == 1 ? gs.lval_bytes : es.src_bytes[1];
+      const int w2 = MODE == 1 ? gs.rval_bytes : es.src_bytes[2];
+      const bool u0 = MODE == 1 ? true : (es.used & 1) != 0;
+      const bool u1 = MODE == 1 ? gs.lval != nullptr : (es.used & 2) != 0;
+      const bool u2 = MODE == 1 ? gs.rval != nullptr : (es.used & 4) != 0;
+#pragma unroll
+      for (int q = 0; q < kExBatch; ++q) {
+        x0[q] = x1[q] = x2[q] = S{};
+        if (okq[q]) {
+          if (u0) x0[q] = load_src<S>(p0, rowq[q], w0);
+          if (u1) x1[q] = load_src<S>(p1, rowq[q], w1);
+          if (u2) x2[q] = load_src<S>(p2, rixq[q], w2);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kExBatch; ++q) {
+        if (!okq[q]) continue;
+        if constexpr (MODE == 2) emit_v(es, gq[q], x0[q], x1[q], x2[q]);
+        else store_w(gs.out, gq[q], gs.out_bytes, compose_v(gs, x0[q], x1[q], x2[q]));
       }
     }
   }
 }
 
+// Grid-stride gather, kExBatch outputs per thread at a time: their index
+// pairs and sources are all loaded before any output is stored (a store
+// may alias a later load as far as the compiler knows).
+template <class S>
 __global__ __launch_bounds__(256) void k_gather(GatherSpec gs, const int32_t* __restrict__ li,
                                                 const int32_t* __restrict__ ri,
                                                 const uint64_t* __restrict__ total_p,
                                                 uint64_t total_cap) {
   const uint64_t total = *total_p < total_cap ? *total_p : total_cap;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
-  for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < total; g += stride)
-    store_w(gs.out, g, gs.out_bytes,
-            compose(gs, static_cast<uint32_t>(li[g]), static_cast<uint32_t>(ri[g])));
+  for (uint64_t g0 = blockIdx.x * 256ull + threadIdx.x; g0 < total; g0 += kExBatch * stride) {
+    uint32_t lq[kExBatch], rq[kExBatch];
+#pragma unroll
+    for (int q = 0; q < kExBatch; ++q) {
+      const uint64_t g = g0 + q * stride;
+      lq[q] = g < total ? static_cast<uint32_t>(li[g]) : 0u;
+      rq[q] = g < total ? static_cast<uint32_t>(ri[g]) : 0u;
+    }
+    S x0[kExBatch], x1[kExBatch], x2[kExBatch];
+#pragma unroll
+    for (int q = 0; q < kExBatch; ++q) {
+      x0[q] = x1[q] = x2[q] = S{};
+      if (g0 + q * stride < total) {
+        x0[q] = load_src<S>(gs.lkey, lq[q], gs.lkey_bytes);
+        if (gs.lval) x1[q] = load_src<S>(gs.lval, lq[q], gs.lval_bytes);
+        if (gs.rval) x2[q] = load_src<S>(gs.rval, rq[q], gs.rval_bytes);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kExBatch; ++q) {
+      const uint64_t g = g0 + q * stride;
+      if (g < total) store_w(gs.out, g, gs.out_bytes, compose_v(gs, x0[q], x1[q], x2[q]));
+    }
+  }
 }
 
+// Scratch of one find_bounds launch over `ntiles` tiles: the merge-path
+// splits, the look-back status words and the tile ticket, each aligned to
+// 256 bytes.
+constexpr uint64_t fb_scratch_bytes(uint64_t ntiles) {
+  return align_up(align_up(8 * (ntiles + 1), 256) + 8 * ntiles, 256) + 4;
+}
+
+// `scratch` is the device memory the launch uses for fb_scratch_bytes():
+// nullptr takes it from the context's workspace (which may grow), anything
+// else is the caller's and is not profiled (profiling creates events).
 template <class K, int NV>
 int find_bounds_nv(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, const void* left,
-                   uint64_t nl, const void* right, uint64_t nr) {
+                   uint64_t nl, const void* right, uint64_t nr, void* scratch) {
   if (nl == 0 || nr == 0) {
     // No matches: every count is zero and every lower bound is 0 (or nr
     // when the right side is empty... which is also 0).
@@ -420,12 +539,12 @@ int find_bounds_nv(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, cons
     return RFX_OK;
   }
   const uint64_t ntiles = (nl + nr + NV - 1) / NV;
-  Carve c(nullptr);
-  c.take<uint64_t>(ntiles + 1);
-  c.take<uint64_t>(ntiles);
-  c.take<uint32_t>(1);
-  RFX_TRY(ensure_workspace(ctx, c.off));
-  Carve w(ctx->ws);
+  const bool own = scratch == nullptr;
+  if (own) {
+    RFX_TRY(ensure_workspace(ctx, fb_scratch_bytes(ntiles)));
+    scratch = ctx->ws;
+  }
+  Carve w(scratch);
   uint64_t* split = w.take<uint64_t>(ntiles + 1);
   uint64_t* status = w.take<uint64_t>(ntiles);
   uint32_t* ticket = w.take<uint32_t>(1);
@@ -435,13 +554,13 @@ int find_bounds_nv(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, cons
   k_fb_partition<K><<<static_cast<uint32_t>((ntiles + 1 + 255) / 256), 256, 0, ctx->stream>>>(
       static_cast<const K*>(left), nl, static_cast<const K*>(right), nr, ntiles, NV, split);
   RFX_LAUNCHED();
-  {
-    RFX_PROF(ctx, "join.find_bounds");
-    k_find_bounds<K, kFbNT, NV><<<static_cast<uint32_t>(ntiles), kFbNT, 0, ctx->stream>>>(
-        static_cast<const K*>(left), nl, static_cast<const K*>(right), nr, split,
-        static_cast<uint32_t>(ntiles), lb, lc, total, status, ticket);
-    RFX_LAUNCHED();
-  }
+  const bool prof = own && ctx->profiling;
+  if (prof) prof_begin(ctx, "join.find_bounds");
+  k_find_bounds<K, kFbNT, NV><<<static_cast<uint32_t>(ntiles), kFbNT, 0, ctx->stream>>>(
+      static_cast<const K*>(left), nl, static_cast<const K*>(right), nr, split,
+      static_cast<uint32_t>(ntiles), lb, lc, total, status, ticket);
+  if (prof) prof_end(ctx);
+  RFX_LAUNCHED();
   return RFX_OK;
 }
 
@@ -449,16 +568,17 @@ int find_bounds_nv(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, cons
 // 16-byte keys always take 2048).
 template <class K>
 int find_bounds_impl(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, const void* left,
-                     uint64_t nl, const void* right, uint64_t nr) {
+                     uint64_t nl, const void* right, uint64_t nr, void* scratch = nullptr) {
   if constexpr (sizeof(K) <= 8) {
-    const int nv = ctx->tune_fb_tile ? ctx->tune_fb_tile : kFbNV;
-    if (nv == 2048) return find_bounds_nv<K, 2048>(ctx, lb, lc, total, left, nl, right, nr);
-    if (nv == 4096) return find_bounds_nv<K, 4096>(ctx, lb, lc, total, left, nl, right, nr);
-    return find_bounds_nv<K, kFbNV>(ctx, lb, lc, total, left, nl, right, nr);
+    const int nv = scratch ? kFbNV : ctx->tune_fb_tile ? ctx->tune_fb_tile : kFbNV;
+    if (nv == 2048) return find_bounds_nv<K, 2048>(ctx, lb, lc, total, left, nl, right, nr, scratch);
+    if (nv == 4096) return find_bounds_nv<K, 4096>(ctx, lb, lc, total, left, nl, right, nr, scratch);
+    return find_bounds_nv<K, kFbNV>(ctx, lb, lc, total, left, nl, right, nr, scratch);
   } else {
-    return find_bounds_nv<K, 2048>(ctx, lb, lc, total, left, nl, right, nr);
+    return find_bounds_nv<K, 2048>(ctx, lb, lc, total, left, nl, right, nr, scratch);
   }
 }
+
 
 template <int NV>
 int expand_nv(rfx_ctx* ctx, int mode, int32_t* li, int32_t* ri, const GatherSpec& gs,
@@ -475,15 +595,27 @@ int expand_nv(rfx_ctx* ctx, int mode, int32_t* li, int32_t* ri, const GatherSpec
       lc, nl, total, total_cap, ntiles, NV, split);
   RFX_LAUNCHED();
   RFX_PROF(ctx, mode == 0 ? "join.expand" : mode == 1 ? "join.expand_gather" : "join.expand_project");
+  // Sources wider than 8 bytes take the 128-bit register form.
+  const bool wide = mode == 1 ? (gs.lkey_bytes == 16 || (gs.lval && gs.lval_bytes == 16) ||
+                                 (gs.rval && gs.rval_bytes == 16))
+                              : (es.src_bytes[0] == 16 || es.src_bytes[1] == 16 ||
+                                 es.src_bytes[2] == 16);
+  const uint32_t grid = static_cast<uint32_t>(ntiles);
   if (mode == 0)
-    k_expand<0, kExNT, NV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
-        lb, lc, nl, total, total_cap, split, li, ri, gs, es);
+    k_expand<0, kExNT, NV, uint64_t><<<grid, kExNT, 0, ctx->stream>>>(lb, lc, nl, total, total_cap,
+                                                                      split, li, ri, gs, es);
+  else if (mode == 1 && !wide)
+    k_expand<1, kExNT, NV, uint64_t><<<grid, kExNT, 0, ctx->stream>>>(lb, lc, nl, total, total_cap,
+                                                                      split, li, ri, gs, es);
   else if (mode == 1)
-    k_expand<1, kExNT, NV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
-        lb, lc, nl, total, total_cap, split, li, ri, gs, es);
+    k_expand<1, kExNT, NV, u128><<<grid, kExNT, 0, ctx->stream>>>(lb, lc, nl, total, total_cap,
+                                                                  split, li, ri, gs, es);
+  else if (!wide)
+    k_expand<2, kExNT, NV, uint64_t><<<grid, kExNT, 0, ctx->stream>>>(lb, lc, nl, total, total_cap,
+                                                                      split, li, ri, gs, es);
   else
-    k_expand<2, kExNT, NV><<<static_cast<uint32_t>(ntiles), kExNT, 0, ctx->stream>>>(
-        lb, lc, nl, total, total_cap, split, li, ri, gs, es);
+    k_expand<2, kExNT, NV, u128><<<grid, kExNT, 0, ctx->stream>>>(lb, lc, nl, total, total_cap,
+                                                                  split, li, ri, gs, es);
   RFX_LAUNCHED();
   return RFX_OK;
 }
@@ -562,8 +694,14 @@ extern "C" int rfx_join_gather(rfx_ctx* ctx, void* out, int out_bytes, const voi
   const uint64_t cap = static_cast<uint64_t>(ctx->device_cus) * 16;
   if (blocks > cap) blocks = cap;
   RFX_PROF(ctx, "join.gather");
-  k_gather<<<static_cast<uint32_t>(blocks), 256, 0, ctx->stream>>>(gs, left_idx, right_idx, total,
-                                                                   total_cap);
+  const bool wide = gs.lkey_bytes == 16 || (gs.lval && gs.lval_bytes == 16) ||
+                    (gs.rval && gs.rval_bytes == 16);
+  if (wide)
+    k_gather<u128><<<static_cast<uint32_t>(blocks), 256, 0, ctx->stream>>>(gs, left_idx, right_idx,
+                                                                           total, total_cap);
+  else
+    k_gather<uint64_t><<<static_cast<uint32_t>(blocks), 256, 0, ctx->stream>>>(
+        gs, left_idx, right_idx, total, total_cap);
   RFX_LAUNCHED();
   return RFX_OK;
 }

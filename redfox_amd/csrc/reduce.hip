@@ -570,10 +570,11 @@ int reduce_by_key_sized(rfx_ctx* ctx, int op, int val_kind, int kb, void* out_ke
   if (n_cap) {
     // No output may overlap an input (as for thrust::reduce_by_key): tiles
     // read their neighbours' keys, and the patch reads keys again after
-    // every run is written.  The inputs span n_cap rows; the outputs at
-    // most min(n_cap, out_cap) runs.
+    // every run is written.  The inputs span n_cap rows; the outputs span
+    // min(n_cap, out_cap) runs when the caller states out_cap, else only
+    // their first row (their true extent depends on the run count).
     const uint64_t vb = val_kind == 1 || val_kind == 2 ? 4 : 8;
-    const uint64_t runs = out_cap < n_cap ? out_cap : n_cap;
+    const uint64_t runs = out_cap == ~0ull ? 1 : out_cap < n_cap ? out_cap : n_cap;
     struct Span { const void* p; uint64_t bytes; };
     const Span outs[3] = {{out_keys, runs * kb}, {out_vals, runs * vb},
                           {val_kind == 3 ? out_counts : nullptr, runs * 4}};

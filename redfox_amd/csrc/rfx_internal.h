@@ -27,6 +27,7 @@ struct rfx_ctx {
   bool capturing = false;     // inside rfx_graph_begin/end: no allocation
   bool ws_borrowed = false;   // ws belongs to the caller: never grown or freed
   int live_graphs = 0;        // instantiated graphs that hold pointers into ws
+  uint64_t* cells = nullptr;  // kCtxCells device u64 cells for scalar operands
   // Tile shapes chosen by rfx_ctx_tune (0: the built-in default).
   int tune_sort_ipt = 0;      // keys per thread of the onesweep tile
   int tune_fb_tile = 0;       // merged keys per find_bounds tile
@@ -64,7 +65,7 @@ inline bool valid_width(int b) {
   return b == 1 || b == 2 || b == 4 || b == 8 || b == 16;
 }
 
-inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+constexpr uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 // Make sure ctx->ws holds `bytes` (synchronises the stream before growing).
 // Refused while graphs captured on the context are alive.
@@ -86,6 +87,12 @@ struct Carve {
     return p;
   }
 };
+
+// The context's device u64 cells (kCtxCells of them, allocated on first
+// use and freed with the context): scalar operands that the operators read
+// on the device, e.g. a row count the Harmony convention passes by value.
+constexpr int kCtxCells = 8;
+int ctx_cells(rfx_ctx* ctx, uint64_t** cells);
 
 // Zero `words` u32 words at `p` on the context stream with a fill kernel
 // (graph-capture safe, unlike hipMemsetAsync).

@@ -187,6 +187,223 @@ def update_size(variable: int, size: int) -> Kernel:
     return Kernel("update_size", UPDATE_SIZE, [(OUT, variable), (IN, size)])
 
 
+def tuple_text(fields) -> str:
+    """A tuple for an rfx1 descriptor: field widths in bits, each an int or
+    (bits, kind) with kind "i", "f" (float) or "p" (string address)."""
+    if not fields:
+        return "-"
+    out = []
+    for f in fields:
+        if isinstance(f, tuple):
+            bits, kind = f
+            out.append(f"{bits}{'' if kind == 'i' else kind}")
+        else:
+            out.append(str(int(f)))
+    return ",".join(out)
+
+
+def descriptor(family: str, **kw) -> bytes:
+    """The rfx1 descriptor of a kernel instance: what Red Fox's generated
+    source bakes in (RelationalAlgebraKernel::cudaSourceRepresentation,
+    RelationalAlgebraKernel.cpp:849-909), written into the kernel's code
+    bytes.  Tuple members (d, d1, a, b, c, e) take field lists; dom, parts
+    and nt ints; map a list of field indices; agg and cmp strings."""
+    parts = ["rfx1", family]
+    for k in ("d", "d1", "a", "b", "c", "e"):
+        if k in kw:
+            parts.append(f"{k}={tuple_text(kw[k])}")
+    for k in ("dom", "parts", "nt"):
+        if k in kw:
+            parts.append(f"{k}={int(kw[k])}")
+    if "map" in kw:
+        parts.append("map=" + ",".join(str(int(i)) for i in kw["map"]))
+    for k in ("agg", "cmp"):
+        if k in kw:
+            parts.append(f"{k}={kw[k]}")
+    return " ".join(parts).encode()
+
+
+def compute(name: str, dims: Sequence[int], *operands: Tuple[int, int], code: bytes = b"") -> Kernel:
+    """A ComputeKernel record: dims are the ctaCount, threadCount and
+    sharedBytes variables (operands 0-2, read by value)."""
+    return Kernel(name, COMPUTE, [(IN, d) for d in dims] + list(operands), code=code)
+
+
+def _bits(fields) -> int:
+    return sum(f[0] if isinstance(f, tuple) else int(f) for f in fields)
+
+
+def _data_type(fields) -> int:
+    """getTupleDataType (RelationalAlgebraCompiler.cpp:494-512): I8..I128
+    by the tuple's total bits."""
+    b = _bits(fields)
+    for code, lim in ((1, 8), (2, 16), (3, 32), (4, 64), (5, 128)):
+        if b <= lim:
+            return code
+    raise ValueError(f"{b}-bit tuple")
+
+
+class Lowering:
+    """Red Fox's lowering of RA nodes into Harmony kernel sequences, written
+    into a Program with rfx1 descriptors in the kernels' code bytes: the
+    same kernels, names, operands and access modes, in the same order, as
+    RelationalAlgebraCompiler.cpp emits them.  Relations are packed tuple
+    buffers (I8 variables holding byte sizes); tuples are field lists as
+    in descriptor()."""
+
+    def __init__(self, prog: Program, first_id: int = 1):
+        self.p = prog
+        self.next_id = first_id
+        self.consts = {}
+
+    def _id(self, family: str) -> str:
+        name = f"{family}{self.next_id}"
+        self.next_id += 1
+        return name
+
+    def const(self, v: int) -> int:
+        """getConstantIntVariable: one I64 variable per value."""
+        if v not in self.consts:
+            self.consts[v] = self.p.u64(v)
+        return self.consts[v]
+
+    def temp_buffer(self) -> int:
+        return self.p.var(I8, 0)
+
+    def temp_int(self) -> int:
+        return self.p.var(I64, 8)
+
+    def _dims(self, cta: int, threads: int):
+        return (self.const(cta), self.const(threads), self.const(0))
+
+    def sort_key(self, ks: List[Kernel], key: int, key_t, size: int) -> None:
+        """addSortKey (:3410-3460)."""
+        fam = "set_mgpusortkey" if _bits(key_t) > 37 else "set_b40csortkey"
+        arg = self.const(_data_type(key_t) if fam == "set_mgpusortkey" else _bits(key_t))
+        ks.append(Kernel(self._id(fam), BINARY, [(INOUT, key), (IN, size), (IN, arg)],
+                         code=descriptor(fam, d=key_t)))
+
+    def sort_pair(self, ks: List[Kernel], key: int, key_t, val: int, val_t, size: int) -> None:
+        """addSortPair (:3462-3539): mergesort past 33 key bits or for wide
+        values, else the LSD radix sort."""
+        kb, vbytes = _bits(key_t), (_bits(val_t) + 7) // 8
+        mgpu = kb > 33 or vbytes > 16 or (kb >= 23 and vbytes > 8) or (kb >= 32 and vbytes > 4)
+        if mgpu:
+            fam, a0 = "set_mgpusortpair", self.const(_data_type(key_t))
+        else:
+            fam, a0 = "set_b40csortpair", self.const(kb)
+        ks.append(Kernel(self._id(fam), BINARY,
+                         [(INOUT, key), (INOUT, val), (IN, size), (IN, a0),
+                          (IN, self.const(_data_type(val_t)))],
+                         code=descriptor(fam, d=key_t, d1=val_t)))
+
+    def _split(self, ks: List[Kernel], src: int, src_t, size: int, keycount: int):
+        key_t, val_t = list(src_t[:keycount]), list(src_t[keycount:])
+        key, val = self.temp_buffer(), self.temp_buffer()
+        ksize, vsize = self.temp_int(), self.temp_int()
+        ks.append(compute(self._id("split_get_result_size"), self._dims(1, 1), (IN, size),
+                          (OUT, ksize), (OUT, vsize),
+                          code=descriptor("split_get_result_size", d=key_t, d1=val_t, a=src_t)))
+        ks.append(resize(key, ksize))
+        ks.append(resize(val, vsize))
+        ks.append(compute(self._id("set_split"), self._dims(350, 256), (INOUT, key), (INOUT, val),
+                          (IN, src), (IN, size),
+                          code=descriptor("set_split", d=key_t, d1=val_t, a=src_t, dom=keycount)))
+        return key, key_t, ksize, val, val_t, vsize
+
+    def join(self, ks: List[Kernel], dest: int, dest_t, a: int, a_t, b: int, b_t, keycount: int,
+             a_sorted: bool = False, b_sorted: bool = False) -> None:
+        """addJoin (:4466-5226)."""
+        lsize, rsize = self.temp_int(), self.temp_int()
+        ks.append(get_size(lsize, a))
+        ks.append(get_size(rsize, b))
+        if keycount < len(a_t):
+            lk, lk_t, lks, lv, lv_t, _ = self._split(ks, a, a_t, lsize, keycount)
+        else:
+            lk, lk_t, lks, lv, lv_t = a, list(a_t), lsize, None, []
+        if keycount < len(b_t):
+            rk, rk_t, rks, rv, rv_t, _ = self._split(ks, b, b_t, rsize, keycount)
+        else:
+            rk, rk_t, rks, rv, rv_t = b, list(b_t), rsize, None, []
+        if not a_sorted:
+            if lv is not None:
+                self.sort_pair(ks, lk, lk_t, lv, lv_t, lks)
+            else:
+                self.sort_key(ks, lk, lk_t, lks)
+        if not b_sorted:
+            if rv is not None:
+                self.sort_pair(ks, rk, rk_t, rv, rv_t, rks)
+            else:
+                self.sort_key(ks, rk, rk_t, rks)
+        lb, lc, lbs = self.temp_buffer(), self.temp_buffer(), self.temp_int()
+        ks.append(compute(self._id("join_get_temp_size"), self._dims(1, 1), (OUT, lbs), (IN, lsize),
+                          code=descriptor("join_get_temp_size", d=a_t)))
+        ks.append(resize(lb, lbs))
+        ks.append(resize(lc, lbs))
+        total = self.temp_int()
+        ks.append(Kernel(self._id("mgpu_join_find_bounds"), BINARY,
+                         [(INOUT, lb), (INOUT, lc), (OUT, total), (IN, lk), (IN, lks), (IN, rk),
+                          (IN, rks)],
+                         code=descriptor("mgpu_join_find_bounds", a=lk_t, b=rk_t)))
+        isz = self.temp_int()
+        ks.append(compute(self._id("join_get_result_size"), self._dims(1, 1), (OUT, isz),
+                          (INOUT, total), code=descriptor("join_get_result_size", d=dest_t)))
+        li, ri = self.temp_buffer(), self.temp_buffer()
+        ks.append(resize(li, isz))
+        ks.append(resize(ri, isz))
+        ks.append(Kernel(self._id("mgpu_join_main"), BINARY,
+                         [(INOUT, li), (INOUT, ri), (IN, isz), (IN, lb), (IN, lc), (IN, lbs),
+                          (IN, rsize)],
+                         code=descriptor("mgpu_join_main")))
+        ks.append(resize(dest, total))
+        ops = [(INOUT, dest), (IN, lk), (IN, li), (IN, rk), (IN, ri), (IN, isz)]
+        if lv is not None:
+            ops.append((IN, lv))
+        if rv is not None:
+            ops.append((IN, rv))
+        ks.append(compute(self._id("mgpu_join_gather"), self._dims(350, 256), *ops,
+                          code=descriptor("mgpu_join_gather", d=dest_t, a=lk_t, b=rk_t, c=lv_t,
+                                          e=rv_t, dom=keycount)))
+
+    def count(self, ks: List[Kernel], dest: int, dest_t, src: int, src_t, domains: Sequence[int],
+              src_sorted: bool = False) -> None:
+        """addAgg's COUNT path (:8335-8641): project the grouping fields,
+        sort them, generate ones, set_count, then merge keys and counts into
+        the destination tuples."""
+        size = self.temp_int()
+        ks.append(get_size(size, src))
+        proj_t = [src_t[i] for i in domains]
+        proj = self.temp_buffer()
+        if (_bits(proj_t) + 7) // 8 != (_bits(src_t) + 7) // 8:
+            ks.append(compute(self._id("project_get_result_size"), self._dims(1, 1), (INOUT, size),
+                              code=descriptor("project_get_result_size", d=proj_t, a=src_t)))
+        ks.append(resize(proj, size))
+        ks.append(compute(self._id("set_project"), self._dims(350, 256), (INOUT, proj), (IN, src),
+                          (IN, size), code=descriptor("set_project", d=proj_t, a=src_t, map=domains)))
+        if not src_sorted:
+            self.sort_key(ks, proj, proj_t, size)
+        gen_t, vsize, gen = [32], self.temp_int(), self.temp_buffer()
+        ks.append(compute(self._id("generate_get_result_size"), self._dims(1, 1), (OUT, vsize),
+                          (IN, size), code=descriptor("generate_get_result_size", d=gen_t, a=proj_t)))
+        ks.append(resize(gen, vsize))
+        ks.append(compute(self._id("set_generate"), self._dims(350, 256), (INOUT, gen), (IN, vsize),
+                          code=descriptor("set_generate", d=gen_t)))
+        rkey, rval = self.temp_buffer(), self.temp_buffer()
+        ks.append(resize(rkey, size))
+        ks.append(resize(rval, vsize))
+        ks.append(Kernel(self._id("set_count"), BINARY,
+                         [(INOUT, rkey), (INOUT, rval), (IN, proj), (INOUT, size), (IN, gen),
+                          (INOUT, vsize)],
+                         code=descriptor("set_count", a=proj_t, b=gen_t, agg="count")))
+        ks.append(compute(self._id("merge_get_result_size"), self._dims(1, 1), (INOUT, size),
+                          (IN, vsize), code=descriptor("merge_get_result_size", d=dest_t, a=gen_t)))
+        ks.append(resize(dest, size))
+        ks.append(compute(self._id("set_merge"), self._dims(350, 256), (INOUT, dest), (IN, rkey),
+                          (IN, size), (IN, rval), (IN, vsize),
+                          code=descriptor("set_merge", d=dest_t, a=proj_t, b=gen_t,
+                                          dom=len(proj_t))))
+
+
 class RfxHirSummary(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 128), ("blocks", ctypes.c_uint32),
                 ("variables", ctypes.c_uint32), ("kernels", ctypes.c_uint32),
@@ -254,4 +471,5 @@ class Hir:
 
 
 __all__ = ["Program", "Hir", "Variable", "Kernel", "Block", "Test", "encode", "binary",
-           "get_size", "resize", "update_size", "RfxError"]
+           "compute", "descriptor", "tuple_text", "Lowering", "get_size", "resize", "update_size",
+           "RfxError"]

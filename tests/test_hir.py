@@ -2,12 +2,14 @@
 liveness-driven allocate/free schedule of Runtime::_scheduleBasicBlock
 (Runtime.cpp:303-445) on hand-worked programs, and rejection of malformed
 files (every truncation and random byte flips of a valid file)."""
+import ctypes
 import struct
 
 import numpy as np
 import pytest
 
 from redfox_amd import hir as H
+from redfox_amd import rfx
 
 
 def straight_program(packed=False):
@@ -139,3 +141,42 @@ def test_random_byte_flips_never_crash():
         except H.RfxError:
             pass
     assert loaded > 0  # some flips only change payload bytes
+
+
+def test_descriptor_text():
+    assert H.descriptor("set_split", d=[32], d1=[32], a=[32, 32], dom=1) == \
+        b"rfx1 set_split d=32 d1=32 a=32,32 dom=1"
+    assert H.descriptor("mgpu_join_gather", d=[40, 24, 24], a=[40], b=[40], c=[24], e=[],
+                        dom=1) == b"rfx1 mgpu_join_gather d=40,24,24 a=40 b=40 c=24 e=- dom=1"
+    assert H.descriptor("set_count", a=[(64, "p")], b=[(64, "f")], agg="count") == \
+        b"rfx1 set_count a=64p b=64f agg=count"
+    assert H.descriptor("set_select", d=[16, 48], cmp="0:lt:300", parts=128, nt=128) == \
+        b"rfx1 set_select d=16,48 parts=128 nt=128 cmp=0:lt:300"
+
+
+def launch(name, kind, code, slots):
+    L = rfx.lib()
+    arr = (ctypes.c_uint64 * max(len(slots), 1))(*slots)
+    st = L.rfx_harmony_launch(name.encode(), kind, code, len(code), arr, len(slots))
+    return st, L.rfx_last_error().decode()
+
+
+def test_launch_refusals_before_any_device_work():
+    # Each is refused while checking the record, before the default context
+    # (and so any device) is touched.
+    st, msg = launch("copy3", 1, b"", [1, 1, 0])
+    assert st == -5 and "compute kernel family 'copy'" in msg
+    st, msg = launch("set_split4", 2, b"", [0] * 4)  # set_split is a compute family
+    assert st == -5 and "binary kernel family 'set_split'" in msg
+    st, msg = launch("set_split4", 1, b"", [350, 256, 0, 1, 2, 3, 4])
+    assert st == -1 and "no rfx1 descriptor" in msg
+    st, msg = launch("set_split4", 1, b"rfx1 set_split d=32 d1=33x a=32,32 dom=1", [1] * 7)
+    assert st == -1 and "bad 'd1=33x'" in msg
+    st, msg = launch("set_split4", 1, b"rfx1 set_merge d=32", [1] * 7)
+    assert st == -1 and "descriptor is for 'set_merge'" in msg
+    st, msg = launch("set_split4", 1, b"rfx1 set_split d=32 d1=32 a=32,32 dom=1", [1, 1])
+    assert st == -1 and "3 or more" in msg
+    st, msg = launch("mgpu_join_find_bounds9", 2, b"", [0] * 7)
+    assert st == -1 and "no rfx1 descriptor" in msg
+    st, msg = launch("set_select2", 1, b"rfx1 set_select d=16 cmp=0:between:3", [1] * 7)
+    assert st == -1 and "cmp=0:between:3" in msg

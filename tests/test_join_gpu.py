@@ -225,3 +225,24 @@ def test_expand_project_matches_gather_then_project(rf, ctx, orc):
     ctx.synchronize()
     assert np.array_equal(K.download(np.uint32, ot), want_k)
     assert np.array_equal(V.download(np.uint64, ot), want_v)
+
+
+@pytest.mark.parametrize("max_run", [3, 300, 700])
+def test_equal_runs_crossing_tiles(rf, ctx, orc, max_run):
+    """Right-side runs of every length from 0 to max_run, so that runs end
+    exactly at, just before and just past a tile's end and past the right
+    keys a tile keeps beyond its end (the global-memory continuation)."""
+    m = 30_000
+    runs = (datagen.splitmix64(97 + max_run, m) % np.uint64(max_run + 1)).astype(np.int64)
+    right = np.repeat(np.arange(m, dtype=np.uint32), runs)
+    left = np.repeat(np.arange(m, dtype=np.uint32), (np.arange(m) % 3) + 0)  # 0, 1 or 2 copies
+    for kb, dt in ((4, np.uint32), (8, np.uint64)):
+        check_join(rf, ctx, orc, left.astype(dt), right.astype(dt), kb)
+
+
+def test_hbm_scale_shape_against_oracle(rf, ctx, orc):
+    """The bench's HBM-scale join shape scaled down 16x (2^20 x 2^22 keys in
+    [0, 2^20)), every lower bound, count and index pair against the oracle."""
+    left = sorted_keys(orc, 4, 1 << 20, 301, 1 << 20)
+    right = sorted_keys(orc, 4, 1 << 22, 302, 1 << 20)
+    assert check_join(rf, ctx, orc, left, right, 4) > (1 << 21)

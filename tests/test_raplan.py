@@ -179,3 +179,25 @@ def test_string_predicates_are_refused():
         raplan.lower_compare(b.comp("eq", b.idx(0, "string"), b.const_int(1)), col)
     with pytest.raises(ValueError, match="string"):
         raplan.lower_exp(b.arith("add", b.idx(0, "string"), b.const_int(1), "int"), col)
+
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def test_reference_no_op_plan_parses():
+    # TPC-H/data/NO_OP.binaryGpu: the plan file the reference's front end
+    # wrote for NO_OP.textGpu (B := A over two 32-bit INT fields).
+    with open(f"{GOLDEN}/no_op.binaryGpu", "rb") as f:
+        g = raplan.parse(f.read())
+    assert g.graphName == "TPC-H/NO_OP.lbb"
+    assert [v.varName for v in g.variables] == ["A", "B"]
+    for v in g.variables:
+        assert [raplan.column_type(t) for t in v.fields] == ["i32", "i32"]
+        assert len(v.keys) == 0
+    assert [s.uniqueIdentifier for s in g.sequences] == [0, 1]
+    s0, s1 = g.sequences
+    assert [c.tag for c in s0.operators] == [5, 4]  # MOVE, GOTO
+    assert (s0.operators[0].move.dest, s0.operators[0].move.src1) == ("B", "A")
+    assert s0.operators[1].jump.target == 1
+    assert [c.tag for c in s1.operators] == [6]  # HALT
+    assert (g.entry, g.exit) == (0, 1)

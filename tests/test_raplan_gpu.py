@@ -203,3 +203,20 @@ def test_string_group_by_plan(rf, ctx):
         assert len(got) == len(a) == len(groups), name
         for m, vs in groups.items():
             assert abs(got[m] - f(vs)) <= 1e-12 * max(1.0, abs(f(vs))), (name, m)
+
+
+def test_reference_no_op_plan_moves_a_to_b(ctx):
+    # The reference's own plan file (TPC-H/data/NO_OP.binaryGpu) over the
+    # reference's own A (TPC-H/data/A: Tuple<32,32> rows, field 0 in the
+    # high word): B must come out byte-equal to A.
+    import os
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    with open(os.path.join(golden, "no_op.binaryGpu"), "rb") as f:
+        p = raplan.Plan(ctx, f.read())
+    a = np.fromfile(os.path.join(golden, "tpch_A.bin"), dtype="<u8")
+    p.set("A", [(a >> 32).astype(np.uint32).view(np.int32),
+                (a & 0xFFFFFFFF).astype(np.uint32).view(np.int32)])
+    assert p.run() == 3
+    f0, f1 = p.get("B")
+    b = (f0.view(np.uint32).astype(np.uint64) << 32) | f1.view(np.uint32).astype(np.uint64)
+    assert b.astype("<u8").tobytes() == a.tobytes()

@@ -228,12 +228,14 @@ def test_fused_sum_and_count(rf, ctx, orc, kb, groups, n):
     reductions, counts exact, sums within 1e-12 of the oracle."""
     keys = sorted_keys(orc, kb, n, 40 + kb, groups)
     vals = datagen.uniform01(41 + kb, n) * 1e4
-    K = rf.DeviceBuffer.from_numpy(keys)
-    V = rf.DeviceBuffer.from_numpy(vals)
+    # Capacity above the row count: the inputs hold n + 12345 rows (the
+    # tail is garbage) and only the *n = n rows on the device count.
+    extra = 12345
+    K = rf.DeviceBuffer.from_numpy(np.concatenate([keys, keys[: extra * kb // keys.itemsize]]))
+    V = rf.DeviceBuffer.from_numpy(np.concatenate([vals, np.full(extra, 7.0)]))
     OK, OV, OC = rf.DeviceBuffer(keys.nbytes), rf.DeviceBuffer(8 * n), rf.DeviceBuffer(4 * n)
     nout, ncell = rf.u64_cell(5), rf.u64_cell(n)
-    # Capacity above the row count: only *n rows count.
-    ctx.reduce_count_by_key_f64(OK, OV, OC, nout, K, V, ncell, n + 12345, kb, 0, out_cap=n)
+    ctx.reduce_count_by_key_f64(OK, OV, OC, nout, K, V, ncell, n + extra, kb, 0, out_cap=n)
     ctx.synchronize()
     g = rf.read_u64(nout)
     fk = OK.download(keys.dtype, g * kb // keys.itemsize)
