@@ -196,6 +196,20 @@ int rfx_join_find_bounds(rfx_ctx* ctx, int32_t* lower_bound,
                          int32_t* left_count, uint64_t* total,
                          const void* left, uint64_t nl, const void* right,
                          uint64_t nr, int key_bytes);
+/* rfx_join_find_bounds without allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as rfx_join_find_bounds; the launch's scratch
+ * (merge-path splits, look-back status words, tile ticket) lives in a
+ * static device array of about 1 MiB per GPU, held until the process
+ * ends, instead of the context's workspace.  At most 2^28 merged keys
+ * (nl + nr) of up to 8 bytes, 2^27 of 16 bytes; more gives RFX_ERANGE
+ * before anything runs.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (concurrent streams or threads) race. */
+int rfx_join_find_bounds_noalloc(rfx_ctx* ctx, int32_t* lower_bound,
+                                 int32_t* left_count, uint64_t* total,
+                                 const void* left, uint64_t nl,
+                                 const void* right, uint64_t nr, int key_bytes);
 /* find_bounds over string keys (find_bounds_string, ModernGPUJoin.cu:
  * 213-312): left/right hold device addresses of NUL-terminated strings,
  * each side sorted in compare_string2's order (:72-97: bytes compared as

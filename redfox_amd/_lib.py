@@ -68,6 +68,7 @@ SIGNATURES = {
     "rfx_sort_pairs": (I, [P, P, P, U64, I, I, I]),
     "rfx_sort_workspace": (U64, [U64, I, I, I]),
     "rfx_join_find_bounds": (I, [P, P, P, P, P, U64, P, U64, I]),
+    "rfx_join_find_bounds_noalloc": (I, [P, P, P, P, P, U64, P, U64, I]),
     "rfx_join_main": (I, [P, P, P, P, U64, P, P, U64]),
     "rfx_join_gather": (I, [P, P, I, P, I, P, I, P, I, P, P, P, U64, U32, U32]),
     "rfx_join_expand_gather": (I, [P, P, I, P, I, P, I, P, I, P, P, U64, P, U64, U32, U32]),

@@ -16,6 +16,8 @@
 //  gather        builds output tuples from the index pairs (gather_*,
 //                ModernGPUJoin.h:43-200); expand_gather fuses the two and
 //                never writes the index columns.
+#include <mutex>
+
 #include "rfx_device.h"
 #include "rfx_internal.h"
 
@@ -579,6 +581,43 @@ int find_bounds_impl(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, co
   }
 }
 
+// ---------------------------------------------------------------------------
+// find_bounds without allocation: the scratch of one launch lives in this
+// file-scope device array, one copy per device, large enough for
+// kFbStaticTiles tiles (2^28 merged keys of up to 8 bytes at the default
+// tile, 2^27 of 16 bytes).  Host code never uses the array's name as an
+// address: each device's copy is looked up once with hipGetSymbolAddress
+// and that pointer is what the kernels get.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kFbStaticTiles = 65536;
+constexpr uint64_t kFbStaticBytes = fb_scratch_bytes(kFbStaticTiles);
+__device__ __align__(256) unsigned char scratch[kFbStaticBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// This device's copy of `scratch` (looked up on the first call per device,
+// with that device current).
+int static_scratch(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(RFX_EINVAL, "find_bounds scratch: device %d", device);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    RFX_HIP(hipGetDevice(&cur));
+    if (cur != device) RFX_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) RFX_HIP(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return fail(RFX_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                  hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return RFX_OK;
+}
 
 template <int NV>
 int expand_nv(rfx_ctx* ctx, int mode, int32_t* li, int32_t* ri, const GatherSpec& gs,
@@ -665,6 +704,34 @@ extern "C" int rfx_join_find_bounds(rfx_ctx* ctx, int32_t* lower_bound, int32_t*
     case 16: return find_bounds_impl<u128>(ctx, lower_bound, left_count, total, left, nl, right, nr);
   }
   return fail(RFX_EINVAL, "rfx_join_find_bounds: key width %d", key_bytes);
+}
+
+extern "C" int rfx_join_find_bounds_noalloc(rfx_ctx* ctx, int32_t* lower_bound,
+                                            int32_t* left_count, uint64_t* total,
+                                            const void* left, uint64_t nl, const void* right,
+                                            uint64_t nr, int key_bytes) {
+  if (!ctx || !total) return fail(RFX_EINVAL, "rfx_join_find_bounds_noalloc: null argument");
+  if (nl && (!lower_bound || !left_count || !left))
+    return fail(RFX_EINVAL, "rfx_join_find_bounds_noalloc: null column");
+  if (nr && !right) return fail(RFX_EINVAL, "rfx_join_find_bounds_noalloc: null right keys");
+  if (!valid_width(key_bytes))
+    return fail(RFX_EINVAL, "rfx_join_find_bounds_noalloc: key width %d", key_bytes);
+  const uint64_t tile = key_bytes == 16 ? 2048 : kFbNV;
+  if (nl + nr > kFbStaticTiles * tile)
+    return fail(RFX_ERANGE,
+                "rfx_join_find_bounds_noalloc: %llu keys exceed its static scratch "
+                "(at most %llu merged keys of %d bytes)",
+                static_cast<unsigned long long>(nl + nr),
+                static_cast<unsigned long long>(kFbStaticTiles * tile), key_bytes);
+  void* s = nullptr;
+  RFX_TRY(static_scratch(ctx->device, &s));
+  switch (key_bytes) {
+    case 1: return find_bounds_impl<uint8_t>(ctx, lower_bound, left_count, total, left, nl, right, nr, s);
+    case 2: return find_bounds_impl<uint16_t>(ctx, lower_bound, left_count, total, left, nl, right, nr, s);
+    case 4: return find_bounds_impl<uint32_t>(ctx, lower_bound, left_count, total, left, nl, right, nr, s);
+    case 8: return find_bounds_impl<uint64_t>(ctx, lower_bound, left_count, total, left, nl, right, nr, s);
+    default: return find_bounds_impl<u128>(ctx, lower_bound, left_count, total, left, nl, right, nr, s);
+  }
 }
 
 extern "C" int rfx_join_main(rfx_ctx* ctx, int32_t* left_idx, int32_t* right_idx,
