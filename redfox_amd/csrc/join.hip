@@ -121,7 +121,7 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
   const uint64_t d0 = static_cast<uint64_t>(tile) * NV;
   const uint64_t d1 = (d0 + NV < na + nb) ? d0 + NV : na + nb;
   const uint64_t a0 = split[tile], a1 = split[tile + 1];
@@ -130,17 +130,21 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
   const uint32_t next = static_cast<uint32_t>(nb - b1 < EXT ? nb - b1 : EXT);
 
   // The tile's A keys, its B keys, then the B keys past its end: every
-  // load in flight before the first LDS write.
+  // load in flight before the first LDS write.  Tile-relative 32-bit
+  // offsets from block-uniform bases keep the address arithmetic scalar.
+  const K* __restrict__ at = a + a0;
+  const K* __restrict__ bt = b + b0;
+  const K* __restrict__ bx = b + b1;
   {
     K v[VT];
     K ev{};
 #pragma unroll
     for (int j = 0; j < VT; ++j) {
       const uint32_t i = tid + j * NT;
-      if (i < nat) v[j] = a[a0 + i];
-      else if (i < nat + nbt) v[j] = b[b0 + (i - nat)];
+      if (i < nat) v[j] = at[i];
+      else if (i < nat + nbt) v[j] = bt[i - nat];
     }
-    if (tid < next) ev = b[b1 + tid];
+    if (tid < next) ev = bx[tid];
 #pragma unroll
     for (int j = 0; j < VT; ++j) {
       const uint32_t i = tid + j * NT;
@@ -152,11 +156,16 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
   const K* sa = keys;
   const K* sb = keys + nat;
   const uint32_t nbx = nbt + next;  // right keys in LDS
+  int32_t* __restrict__ lbt = lower_bound + a0;
 
-  // Left key i = tid + j * NT; JA (block-uniform) bounds j.
+  // Left key i = tid + j * NT; JA (block-uniform) bounds j.  The lower
+  // bound by a fixed-step binary search of the tile's right keys; the
+  // equal run after it by a fixed-step search of the next kRunWin right
+  // keys, widened to all of them only for a wave in which some run is
+  // longer.
+  constexpr uint32_t kRunWin = 64;
   const uint32_t JA = (nat + NT - 1) / NT;
   const uint32_t top_lo = nbt ? 1u << (31 - __builtin_clz(nbt)) : 0u;
-  const uint32_t top_up = nbx ? 1u << (31 - __builtin_clz(nbx)) : 0u;
   uint32_t cnt[VT];
 #pragma unroll
   for (int j = 0; j < VT; ++j) cnt[j] = 0;
@@ -169,17 +178,37 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
       for (int q = 0; q < 4; ++q) {
         const uint32_t i = tid + (jb + q) * NT;
         kv[q] = sa[i < nat ? i : nat - 1];
-        lo[q] = up[q] = 0;
+        lo[q] = 0;
       }
       for (uint32_t step = top_lo; step; step >>= 1) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
           if (lo[q] + step <= nbt && lt(sb[lo[q] + step - 1], kv[q])) lo[q] += step;
       }
-      for (uint32_t step = top_up; step; step >>= 1) {
+      // sb[lo, up) == key throughout: sb[lo..] >= key, so the upper bound
+      // is the end of the run of equal keys.
+      bool longer = false;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (up[q] + step <= nbx && !lt(kv[q], sb[up[q] + step - 1])) up[q] += step;
+      for (int q = 0; q < 4; ++q) {
+        up[q] = lo[q];
+        const uint32_t lim = nbx - lo[q] < kRunWin ? nbx : lo[q] + kRunWin;
+        for (uint32_t step = kRunWin; step; step >>= 1)
+          if (up[q] + step <= lim && !lt(kv[q], sb[up[q] + step - 1])) up[q] += step;
+        longer |= up[q] == lo[q] + kRunWin && up[q] < nbx;
+      }
+      if (ballot(longer)) {
+        // Some run fills the window: finish those from its end.
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (up[q] == lo[q] + kRunWin && up[q] < nbx) {
+            uint32_t u = 0;
+            const uint32_t rem = nbx - up[q];
+            const uint32_t top = 1u << (31 - __builtin_clz(rem));
+            for (uint32_t step = top; step; step >>= 1)
+              if (u + step <= rem && !lt(kv[q], sb[up[q] + u + step - 1])) u += step;
+            up[q] += u;
+          }
+        }
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -187,9 +216,9 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
         if (i < nat) {
           uint64_t c = up[q] - lo[q];
           // A run reaching past the keys in LDS continues in global memory.
-          const uint64_t bx = b1 + next;
-          if (up[q] == nbx && bx < nb && eq(b[bx], kv[q])) {
-            uint64_t l = bx + 1, st = 1, h;
+          const uint64_t bxe = b1 + next;
+          if (up[q] == nbx && bxe < nb && eq(b[bxe], kv[q])) {
+            uint64_t l = bxe + 1, st = 1, h;
             while (true) {  // invariant: b[l - 1] == key
               h = l + st - 1;
               if (h >= nb) { h = nb; break; }
@@ -202,9 +231,9 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
               if (eq(b[mid], kv[q])) l = mid + 1;
               else h = mid;
             }
-            c += l - bx;
+            c += l - bxe;
           }
-          lower_bound[a0 + i] = static_cast<int32_t>(b0 + lo[q]);
+          lbt[i] = static_cast<int32_t>(b0 + lo[q]);
           cnt[jb + q] = static_cast<uint32_t>(c);
         }
       }
@@ -252,8 +281,9 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
   }
   __syncthreads();
   const uint64_t prefix = s_prefix;
+  int32_t* __restrict__ lct = left_count + a0;
   for (uint32_t i = tid; i < nat; i += NT)
-    left_count[a0 + i] = static_cast<int32_t>(prefix + s_cnt[cnt_slot(i)]);
+    lct[i] = static_cast<int32_t>(prefix + s_cnt[cnt_slot(i)]);
   if (tile == ntiles - 1 && tid == 0) *total = prefix + tile_sum;
 }
 

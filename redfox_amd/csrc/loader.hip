@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kLNT) void k_tbl_rows(const uint8_t* __restrict__ t
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
   const uint64_t at = static_cast<uint64_t>(tile) * kLTile + static_cast<uint64_t>(tid) * kLBytes;
   // The thread's 64 bytes as sixteen words; bytes past the end are not
   // newlines.

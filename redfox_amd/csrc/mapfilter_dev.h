@@ -311,7 +311,7 @@ __device__ __forceinline__ void mapfilter_tile(Filter filter, Compute compute, S
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
   const uint64_t base = static_cast<uint64_t>(tile) * kMTile;
   uint32_t keep = 0;
 #pragma unroll 1

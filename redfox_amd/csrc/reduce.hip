@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kBNT) void k_rbk_onepass(
   uint64_t* hst = st + 2;
   if (threadIdx.x == 0) s_tile = atomicAdd(reinterpret_cast<uint32_t*>(st), 1u);
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
   const uint64_t n = rows_of(z);
   const uint64_t base = static_cast<uint64_t>(tile) * TILE;
   if (base >= n) return;  // uniform over the block

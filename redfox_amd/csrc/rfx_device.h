@@ -48,6 +48,26 @@ template <class K>
 __device__ __forceinline__ uint32_t digit_of(K k, uint32_t shift, uint32_t dmask) {
   return static_cast<uint32_t>(static_cast<uint64_t>(k) >> shift) & dmask;
 }
+// Keys of at most 32 bits: one 32-bit shift (shift < 32 for them).
+template <>
+__device__ __forceinline__ uint32_t digit_of<uint32_t>(uint32_t k, uint32_t shift, uint32_t dmask) {
+  return (k >> shift) & dmask;
+}
+template <>
+__device__ __forceinline__ uint32_t digit_of<uint16_t>(uint16_t k, uint32_t shift, uint32_t dmask) {
+  return (static_cast<uint32_t>(k) >> shift) & dmask;
+}
+template <>
+__device__ __forceinline__ uint32_t digit_of<uint8_t>(uint8_t k, uint32_t shift, uint32_t dmask) {
+  return (static_cast<uint32_t>(k) >> shift) & dmask;
+}
+// 64-bit keys: the 32-bit half holding the digit (shift is a multiple of
+// 8, so a digit never straddles the halves).
+template <>
+__device__ __forceinline__ uint32_t digit_of<uint64_t>(uint64_t k, uint32_t shift, uint32_t dmask) {
+  const uint32_t h = shift >= 32 ? static_cast<uint32_t>(k >> 32) : static_cast<uint32_t>(k);
+  return (h >> (shift & 31)) & dmask;
+}
 template <>
 __device__ __forceinline__ uint32_t digit_of<u128>(u128 k, uint32_t shift, uint32_t dmask) {
   uint64_t w = shift >= 64 ? (k.hi >> (shift - 64)) : (k.lo >> shift);

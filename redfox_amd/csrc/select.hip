@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kFNT) void k_select(const K* __restrict__ in, uint6
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
   const uint64_t base = static_cast<uint64_t>(tile) * TILE;
   // Step j of wave w covers rows base + j*kFNT + w*64 .. +63.
   K r[VT];

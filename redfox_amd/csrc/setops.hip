@@ -44,6 +44,33 @@ __device__ __forceinline__ uint64_t mp_lower(const K* a, uint64_t na, const K* b
   return lo;
 }
 
+// Element shuffles across the wave (lane - delta / lane + delta; a lane
+// with no such neighbour gets its own value).
+template <class K>
+__device__ __forceinline__ K shfl_up_elem(K v, int delta) {
+  if constexpr (sizeof(K) <= 4) {
+    return static_cast<K>(__shfl_up(static_cast<uint32_t>(v), delta, kWave));
+  } else if constexpr (sizeof(K) == 8) {
+    const uint32_t lo = __shfl_up(static_cast<uint32_t>(v), delta, kWave);
+    const uint32_t hi = __shfl_up(static_cast<uint32_t>(v >> 32), delta, kWave);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+  } else {
+    return K{shfl_up_elem<uint64_t>(v.lo, delta), shfl_up_elem<uint64_t>(v.hi, delta)};
+  }
+}
+template <class K>
+__device__ __forceinline__ K shfl_down_elem(K v, int delta) {
+  if constexpr (sizeof(K) <= 4) {
+    return static_cast<K>(__shfl_down(static_cast<uint32_t>(v), delta, kWave));
+  } else if constexpr (sizeof(K) == 8) {
+    const uint32_t lo = __shfl_down(static_cast<uint32_t>(v), delta, kWave);
+    const uint32_t hi = __shfl_down(static_cast<uint32_t>(v >> 32), delta, kWave);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+  } else {
+    return K{shfl_down_elem<uint64_t>(v.lo, delta), shfl_down_elem<uint64_t>(v.hi, delta)};
+  }
+}
+
 template <class K>
 __global__ __launch_bounds__(256) void k_set_partition(const K* __restrict__ a, uint64_t na,
                                                        const K* __restrict__ b, uint64_t nb,
@@ -143,11 +170,13 @@ __global__ __launch_bounds__(NT) void k_set_op(
   __shared__ uint32_t scan_tmp[NT / kWave + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
-  __shared__ K s_edge[3];  // a[a0 - 1], b[b0 - 1], b[b1] where they exist
+  __shared__ K s_edge[4];  // a[a0 - 1], b[b0 - 1], b[b1], a[a1] where they exist
+  __shared__ K s_wlast[NT / kWave], s_wfirst[NT / kWave];
+  __shared__ bool s_wlast_a[NT / kWave];
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
   const uint64_t d0 = static_cast<uint64_t>(tile) * NV;
   const uint64_t d1 = (d0 + NV < na + nb) ? d0 + NV : na + nb;
   const uint64_t a0 = split[tile], a1 = split[tile + 1];
@@ -160,6 +189,7 @@ __global__ __launch_bounds__(NT) void k_set_op(
   if (tid == 0 && a0 > 0) s_edge[0] = a[a0 - 1];
   if (tid == 1 && b0 > 0) s_edge[1] = b[b0 - 1];
   if (tid == 2 && b1 < nb) s_edge[2] = b[b1];
+  if (tid == 3 && a1 < na) s_edge[3] = a[a1];
   if constexpr (sizeof(K) <= 8) {
     K v[VT];
 #pragma unroll
@@ -183,12 +213,9 @@ __global__ __launch_bounds__(NT) void k_set_op(
   const Side<K> GB{b, nullptr, nb, 0, 0};
 
   // Thread t merges the tile's items [t*VT, t*VT + VT): one merge-path
-  // search on its diagonal (A wins ties), then VT serial steps.  The merge
-  // pointer into the other input at the moment an item is taken is its
-  // bound there: for an A item the count of B items below it, for a B
-  // item the count of A items at or below it.
+  // search on its diagonal (A wins ties), then VT serial steps.
   const uint32_t diag = tid * VT < nm ? tid * VT : nm;
-  uint32_t ai, bi;
+  uint32_t ai0, bi0;
   {
     uint32_t lo = diag > nbt ? diag - nbt : 0u;
     uint32_t hi = diag < nat ? diag : nat;
@@ -197,49 +224,127 @@ __global__ __launch_bounds__(NT) void k_set_op(
       if (!lt(sb[diag - 1 - mid], sa[mid])) lo = mid + 1;
       else hi = mid;
     }
-    ai = lo;
-    bi = diag - lo;
+    ai0 = lo;
+    bi0 = diag - lo;
   }
+  const uint32_t nk = nm - diag < static_cast<uint32_t>(VT) ? nm - diag : static_cast<uint32_t>(VT);
+
+  // Fast path, for tiles in which no value has two copies in one input
+  // (checked below): the merged items alone decide.  Equal items of A
+  // and B are adjacent in merged order, A's first, so a B item is in A
+  // exactly when the item before it equals it, and an A item is in B
+  // exactly when the item after it does.
   K mv[VT];
-  uint32_t kmask = 0;
-  for (int k = 0; k < VT; ++k) {
-    if (diag + k >= nm) break;
-    const bool take_a = bi >= nbt || (ai < nat && !lt(sb[bi], sa[ai]));
-    bool keep;
-    if (take_a) {
-      const K x = sa[ai];
-      keep = true;
-      if constexpr (OP != RFX_SET_UNION) {
-        // Copies of x in A before this one, and in B from bi on (equal
-        // runs that cross the tile edge continue in global memory).
-        const uint32_t before = lds_run_back(sa, ai, x);
-        uint64_t rank = before;
-        if (before == ai && a0 > 0 && eq(s_edge[0], x)) rank += GA.run_bwd(a0, x);
-        const uint32_t in_b = lds_run_fwd(sb, bi, nbt, x);
-        uint64_t cnt = in_b;
-        if (bi + in_b == nbt && b1 < nb && eq(s_edge[2], x)) cnt += GB.run_fwd(b1, x);
-        keep = (OP == RFX_SET_DIFFERENCE) ? rank >= cnt : rank < cnt;
+  mv[0] = K{};  // read (not used) by threads past the tile's items
+  uint32_t amask = 0;
+  {
+    uint32_t ai = ai0, bi = bi0;
+#pragma unroll
+    for (int k = 0; k < VT; ++k) {
+      if (static_cast<uint32_t>(k) < nk) {
+        const bool take_a = bi >= nbt || (ai < nat && !lt(sb[bi], sa[ai]));
+        mv[k] = take_a ? sa[ai] : sb[bi];
+        amask |= static_cast<uint32_t>(take_a) << k;
+        ai += take_a;
+        bi += !take_a;
       }
-      mv[k] = x;
-      ++ai;
-    } else {
-      const K y = sb[bi];
-      keep = false;
-      if constexpr (OP == RFX_SET_UNION) {
-        // Copies of y in B before this one and in A up to ai (none of A's
-        // copies lie at or past a1: they would be taken before y).
-        const uint32_t before = lds_run_back(sb, bi, y);
-        uint64_t rank = before;
-        if (before == bi && b0 > 0 && eq(s_edge[1], y)) rank += GB.run_bwd(b0, y);
-        const uint32_t in_a = lds_run_back(sa, ai, y);
-        uint64_t cnt = in_a;
-        if (in_a == ai && a0 > 0 && eq(s_edge[0], y)) cnt += GA.run_bwd(a0, y);
-        keep = rank >= cnt;
-      }
-      mv[k] = y;
-      ++bi;
     }
-    kmask |= static_cast<uint32_t>(keep) << k;
+  }
+  // Neighbours across threads: the item before my first (and its source)
+  // and the item after my last.
+  K my_last = mv[0];
+#pragma unroll
+  for (int k = 1; k < VT; ++k)
+    if (static_cast<uint32_t>(k) < nk) my_last = mv[k];
+  const bool last_a = nk && ((amask >> (nk - 1)) & 1u);
+  const uint32_t lane = lane_id(), w = tid / kWave;
+  if (lane == kWave - 1) { s_wlast[w] = my_last; s_wlast_a[w] = last_a; }
+  if (lane == 0) s_wfirst[w] = mv[0];
+  bool dup = false;
+  if (tid == 0) {
+    // A value with copies on both sides of a tile edge.
+    if (nat && a0 > 0 && eq(s_edge[0], sa[0])) dup = true;
+    if (nbt && b0 > 0 && eq(s_edge[1], sb[0])) dup = true;
+    if (nat && a1 < na && eq(s_edge[3], sa[nat - 1])) dup = true;
+    if (nbt && b1 < nb && eq(s_edge[2], sb[nbt - 1])) dup = true;
+  }
+  __syncthreads();
+  K prev = shfl_up_elem(my_last, 1);
+  bool prev_a = __shfl_up(static_cast<int>(last_a), 1, kWave) != 0;
+  K next = shfl_down_elem(mv[0], 1);
+  if (lane == 0 && w > 0) { prev = s_wlast[w - 1]; prev_a = s_wlast_a[w - 1]; }
+  if (lane == kWave - 1 && w + 1 < NT / kWave) next = s_wfirst[w + 1];
+  const bool has_prev = diag > 0 || a0 > 0;        // an item before my first
+  if (diag == 0 && a0 > 0) { prev = s_edge[0]; prev_a = true; }
+  const bool last_item = diag + nk == nm;          // my last item ends the tile
+  const bool has_next = !last_item || b1 < nb;
+  if (last_item) next = s_edge[2];
+  // Two equal items of one input are adjacent in merged order.
+  if (nk) {
+    if (diag > 0 && eq(prev, mv[0]) && prev_a == (amask & 1u)) dup = true;
+#pragma unroll
+    for (int k = 1; k < VT; ++k)
+      if (static_cast<uint32_t>(k) < nk && eq(mv[k], mv[k - 1]) &&
+          ((amask >> k) & 1u) == ((amask >> (k - 1)) & 1u))
+        dup = true;
+  }
+  uint32_t kmask = 0;
+  if (!__syncthreads_or(dup)) {
+#pragma unroll
+    for (int k = 0; k < VT; ++k) {
+      if (static_cast<uint32_t>(k) >= nk) continue;
+      const bool is_a = (amask >> k) & 1u;
+      const K before = k == 0 ? prev : mv[k > 0 ? k - 1 : 0];
+      const bool has_before = k > 0 || has_prev;
+      const K after = k + 1 < static_cast<int>(nk) ? mv[k + 1 < VT ? k + 1 : k] : next;
+      const bool has_after = k + 1 < static_cast<int>(nk) || has_next;
+      bool keep;
+      if constexpr (OP == RFX_SET_UNION) keep = is_a || !(has_before && eq(before, mv[k]));
+      else if constexpr (OP == RFX_SET_INTERSECTION) keep = is_a && has_after && eq(after, mv[k]);
+      else keep = is_a && !(has_after && eq(after, mv[k]));
+      kmask |= static_cast<uint32_t>(keep) << k;
+    }
+  } else {
+    // General path: multiset counts from equal runs, in LDS and, for runs
+    // crossing a tile edge, in global memory.
+    uint32_t ai = ai0, bi = bi0;
+    for (int k = 0; k < VT; ++k) {
+      if (static_cast<uint32_t>(k) >= nk) break;
+      const bool take_a = bi >= nbt || (ai < nat && !lt(sb[bi], sa[ai]));
+      bool keep;
+      if (take_a) {
+        const K x = sa[ai];
+        keep = true;
+        if constexpr (OP != RFX_SET_UNION) {
+          // Copies of x in A before this one, and in B from bi on (equal
+          // runs that cross the tile edge continue in global memory).
+          const uint32_t before = lds_run_back(sa, ai, x);
+          uint64_t rank = before;
+          if (before == ai && a0 > 0 && eq(s_edge[0], x)) rank += GA.run_bwd(a0, x);
+          const uint32_t in_b = lds_run_fwd(sb, bi, nbt, x);
+          uint64_t cnt = in_b;
+          if (bi + in_b == nbt && b1 < nb && eq(s_edge[2], x)) cnt += GB.run_fwd(b1, x);
+          keep = (OP == RFX_SET_DIFFERENCE) ? rank >= cnt : rank < cnt;
+        }
+        ++ai;
+      } else {
+        const K y = sb[bi];
+        keep = false;
+        if constexpr (OP == RFX_SET_UNION) {
+          // Copies of y in B before this one and in A up to ai (none of
+          // A's copies lie at or past a1: they would be taken before y).
+          const uint32_t before = lds_run_back(sb, bi, y);
+          uint64_t rank = before;
+          if (before == bi && b0 > 0 && eq(s_edge[1], y)) rank += GB.run_bwd(b0, y);
+          const uint32_t in_a = lds_run_back(sa, ai, y);
+          uint64_t cnt = in_a;
+          if (in_a == ai && a0 > 0 && eq(s_edge[0], y)) cnt += GA.run_bwd(a0, y);
+          keep = rank >= cnt;
+        }
+        ++bi;
+      }
+      kmask |= static_cast<uint32_t>(keep) << k;
+    }
   }
 
   // Kept items in merged order: scan of the per-thread counts, then the
@@ -287,7 +392,7 @@ __global__ __launch_bounds__(kSNT) void k_unique(const K* __restrict__ in, uint6
   const uint32_t tid = threadIdx.x;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
   const uint64_t base = static_cast<uint64_t>(tile) * kSNV;
   const uint32_t cnt = static_cast<uint32_t>((n - base) < kSNV ? (n - base) : kSNV);
   for (uint32_t i = tid; i < cnt + 1; i += kSNT) {

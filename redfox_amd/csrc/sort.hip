@@ -15,6 +15,8 @@
 // skipped; which of the two buffers holds the data is tracked in a device
 // word, and a final kernel copies back only if it is the pong buffer.  No
 // host round trip, so a sort can be captured into a HIP graph.
+#include <type_traits>
+
 #include "rfx_device.h"
 #include "rfx_internal.h"
 
@@ -175,7 +177,6 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
 
   __shared__ uint32_t s_tile;
   __shared__ uint32_t wcnt[NW][kBins];
-  __shared__ uint32_t bstart[kBins];
   __shared__ uint32_t gdelta[kBins];
   __shared__ uint32_t scan_tmp[NW + 1];
   __shared__ __attribute__((aligned(16))) unsigned char rbuf[RB];
@@ -190,133 +191,155 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
   if (tid == 0) s_tile = atomicAdd(&ctrs[pass * 2], 1u);
   for (uint32_t i = tid; i < NW * kBins; i += NT) (&wcnt[0][0])[i] = 0;
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
   const uint64_t tile_base = static_cast<uint64_t>(tile) * TILE;
   const uint64_t wave_base = tile_base + static_cast<uint64_t>(w) * kWave * IPT;
-
-  // Keys only: values are read after the keys are scattered, which keeps
-  // the ranking phase's register footprint small.
-  K keys[IPT];
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
-    keys[j] = idx < n ? kin[idx] : K{};
-  }
-
-  // Stable rank within the wave: lanes sharing a digit are found by one
-  // ballot per digit bit; the highest such lane bumps the wave counter.
-  // (An LDS atomic per group leader, issued for all keys before any result
-  // is used, measured slower on MI355X: 1.67 against 1.45 ms per pass.)
-  uint32_t pos[IPT];
-#pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
-    const bool valid = idx < n;
-    const uint32_t d = digit_of(keys[j], shift, dmask);
-    uint64_t m = ballot(valid);
-    // Bits above a narrow last digit are zero in every lane: their ballot
-    // is empty and the update below leaves m unchanged, so all 8 bits run
-    // unconditionally.  x is all ones where the lane's bit is set, and
-    // ~(bb ^ x) selects bb there and ~bb elsewhere.
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint32_t x = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
-      const uint64_t bb = ballot(x != 0u);
-      const uint64_t xx = (static_cast<uint64_t>(x) << 32) | x;
-      m &= ~(bb ^ xx);
-    }
-    const uint32_t below = mbcnt(m);
-    const uint32_t cnt = __popcll(m);
-    uint32_t prior = 0;
-    if (valid) {
-      prior = wcnt[w][d];
-      if (below == cnt - 1) wcnt[w][d] = prior + cnt;
-    }
-    pos[j] = prior + below;
-  }
-  __syncthreads();
-
-  // Per-digit tile totals and wave prefixes; publish the tile aggregate.
-  uint32_t total_d = 0;
   uint32_t* st = status + (static_cast<uint64_t>(pass) * ntiles) * kBins;
-  if (tid < kBins) {
-    uint32_t run = 0;
-#pragma unroll
-    for (int ww = 0; ww < NW; ++ww) {
-      const uint32_t c = wcnt[ww][tid];
-      wcnt[ww][tid] = run;
-      run += c;
-    }
-    total_d = run;
-    status_store(st + static_cast<uint64_t>(tile) * kBins + tid,
-                 (tile == 0 ? kFlagIncl : kFlagAgg) | total_d);
-  }
-  uint32_t all;
-  const uint32_t bs = block_excl_scan<NT>(tid < kBins ? total_d : 0u, scan_tmp, &all);
-  if (tid < kBins) bstart[tid] = bs;
-  __syncthreads();
 
-  // Work that does not need the global offsets goes before the look-back so
-  // it overlaps the wait: the LDS reorder of the keys and the value loads.
-  K* kbuf = reinterpret_cast<K*>(rbuf);
+  // Every tile but the last is full: its copy of the body has no per-key
+  // bounds checks.
+  auto body = [&](auto full_tag) {
+    constexpr bool FULL = decltype(full_tag)::value;
+    // Keys only: values are read after the keys are ranked, which keeps
+    // the ranking phase's register footprint small.
+    K keys[IPT];
 #pragma unroll
-  for (int j = 0; j < IPT; ++j) {
-    const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
-    if (idx < n) {
+    for (int j = 0; j < IPT; ++j) {
+      const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+      if (FULL) keys[j] = kin[idx];
+      else keys[j] = idx < n ? kin[idx] : K{};
+    }
+
+    // Stable rank within the wave: lanes sharing a digit are found by one
+    // ballot per digit bit; the highest such lane bumps the wave counter.
+    // (An LDS atomic per group leader, issued for all keys before any
+    // result is used, measured slower on MI355X: 1.67 against 1.45 ms per
+    // pass.)
+    uint32_t pos[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+      const bool valid = FULL || idx < n;
       const uint32_t d = digit_of(keys[j], shift, dmask);
-      pos[j] += bstart[d] + wcnt[w][d];
-      kbuf[pos[j]] = keys[j];
-    }
-  }
-  V vals[IPT];
-  if constexpr (VB > 0) {
+      // acc collects, per lane, the lanes whose digit differs in some bit:
+      // x is the lane's bit b as 0 or all ones, and acc | (bb ^ x) is one
+      // bitop3 per 32-bit half (table 0xDE with acc in the middle operand
+      // is the same under either operand order of the truth table).  Bits
+      // above a narrow last digit are zero everywhere and change nothing.
+      uint32_t acc_lo = 0, acc_hi = 0;
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
-      if (idx < n) vals[j] = vin[idx];
-    }
-  }
-  if (tid < kBins) {
-    uint32_t excl = 0;
-    if (tile > 0) {
-      excl = lookback_window<RFX_SORT_LBW>(st, tile, kBins, tid);
-      status_store(st + static_cast<uint64_t>(tile) * kBins + tid, kFlagIncl | (excl + total_d));
-    }
-    gdelta[tid] = base[pass * kBins + tid] + excl - bs;
-  }
-  __syncthreads();
-
-  const uint32_t tile_n =
-      static_cast<uint32_t>((n - tile_base) < TILE ? (n - tile_base) : TILE);
-#pragma unroll
-  for (int k = 0; k < IPT; ++k) {
-    const uint32_t i = k * NT + tid;
-    if (i < tile_n) {
-      const K key = kbuf[i];
-      const uint32_t d = digit_of(key, shift, dmask);
-      if constexpr (VB > 0) sdig[i] = static_cast<uint8_t>(d);
-      const uint32_t dst = gdelta[d] + i;
-      if (dst < n) kout[dst] = key;  // always true when ranks are right
-    }
-  }
-  if constexpr (VB > 0) {
-    __syncthreads();  // every key is out of kbuf
-    V* vbuf = reinterpret_cast<V*>(rbuf);
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
-      if (idx < n) vbuf[pos[j]] = vals[j];
+      for (int b = 0; b < 8; ++b) {
+        const uint32_t x = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(d), b, 1));
+        // The ballot of x itself (a plain ballot(x != 0) is rewritten into
+        // a second shift-and-compare of d); 33 is the not-equal predicate.
+        // Lanes outside exec give 0, as a ballot does.
+        const uint64_t bb = __builtin_amdgcn_uicmp(x, 0u, 33);
+        acc_lo = __builtin_amdgcn_bitop3_b32(static_cast<uint32_t>(bb), acc_lo, x, 0xDE);
+        acc_hi = __builtin_amdgcn_bitop3_b32(static_cast<uint32_t>(bb >> 32), acc_hi, x, 0xDE);
+      }
+      const uint64_t m =
+          (FULL ? ~0ull : ballot(valid)) & ~((static_cast<uint64_t>(acc_hi) << 32) | acc_lo);
+      const uint32_t below = mbcnt(m);
+      const uint32_t cnt = __popcll(m);
+      uint32_t prior = 0;
+      if (valid) {
+        prior = wcnt[w][d];
+        if (below == cnt - 1) wcnt[w][d] = prior + cnt;
+      }
+      pos[j] = prior + below;
     }
     __syncthreads();
+
+    // Per-digit tile totals and wave prefixes; publish the tile aggregate.
+    // wcnt[w][d] becomes the tile position of wave w's first key of digit
+    // d: the digit's start in the tile plus the waves before w.
+    uint32_t total_d = 0;
+    uint32_t wpre[NW];
+    if (tid < kBins) {
+      uint32_t run = 0;
+#pragma unroll
+      for (int ww = 0; ww < NW; ++ww) {
+        wpre[ww] = run;
+        run += wcnt[ww][tid];
+      }
+      total_d = run;
+      status_store(st + static_cast<uint64_t>(tile) * kBins + tid,
+                   (tile == 0 ? kFlagIncl : kFlagAgg) | total_d);
+    }
+    uint32_t all;
+    const uint32_t bs = block_excl_scan<NT>(tid < kBins ? total_d : 0u, scan_tmp, &all);
+    if (tid < kBins) {
+#pragma unroll
+      for (int ww = 0; ww < NW; ++ww) wcnt[ww][tid] = bs + wpre[ww];
+    }
+    __syncthreads();
+
+    // Work that does not need the global offsets goes before the look-back
+    // so it overlaps the wait: the LDS reorder of the keys and the value
+    // loads.
+    K* kbuf = reinterpret_cast<K*>(rbuf);
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+      if (FULL || idx < n) {
+        const uint32_t d = digit_of(keys[j], shift, dmask);
+        pos[j] += wcnt[w][d];
+        kbuf[pos[j]] = keys[j];
+      }
+    }
+    V vals[IPT];
+    if constexpr (VB > 0) {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+        if (FULL || idx < n) vals[j] = vin[idx];
+      }
+    }
+    if (tid < kBins) {
+      uint32_t excl = 0;
+      if (tile > 0) {
+        excl = lookback_window<RFX_SORT_LBW>(st, tile, kBins, tid);
+        status_store(st + static_cast<uint64_t>(tile) * kBins + tid, kFlagIncl | (excl + total_d));
+      }
+      gdelta[tid] = base[pass * kBins + tid] + excl - bs;
+    }
+    __syncthreads();
+
+    const uint32_t tile_n =
+        FULL ? static_cast<uint32_t>(TILE)
+             : static_cast<uint32_t>((n - tile_base) < TILE ? (n - tile_base) : TILE);
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
       const uint32_t i = k * NT + tid;
-      if (i < tile_n) {
-        const uint32_t dst = gdelta[sdig[i]] + i;
-        if (dst < n) vout[dst] = vbuf[i];
+      if (FULL || i < tile_n) {
+        const K key = kbuf[i];
+        const uint32_t d = digit_of(key, shift, dmask);
+        if constexpr (VB > 0) sdig[i] = static_cast<uint8_t>(d);
+        const uint32_t dst = gdelta[d] + i;
+        if (dst < n) kout[dst] = key;  // always true when ranks are right
       }
     }
-  }
+    if constexpr (VB > 0) {
+      __syncthreads();  // every key is out of kbuf
+      V* vbuf = reinterpret_cast<V*>(rbuf);
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
+        if (FULL || idx < n) vbuf[pos[j]] = vals[j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < IPT; ++k) {
+        const uint32_t i = k * NT + tid;
+        if (FULL || i < tile_n) {
+          const uint32_t dst = gdelta[sdig[i]] + i;
+          if (dst < n) vout[dst] = vbuf[i];
+        }
+      }
+    }
+  };
+  if (tile_base + TILE <= n) body(std::true_type{});
+  else body(std::false_type{});
 
   // The last tile to finish flips the buffer selector.  Every block read
   // *sel before arriving here, so nobody reads the new value this pass.

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kSNT) void k_count_scan(int32_t* __restrict__ count
   const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
   const uint64_t at = static_cast<uint64_t>(tile) * kCTile + static_cast<uint64_t>(tid) * kCVT;
   uint64_t v[kCVT], sum = 0;
 #pragma unroll
