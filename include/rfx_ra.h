@@ -469,7 +469,12 @@ typedef struct rfx_serve_stats {
 int rfx_server_create(rfx_server** srv, int device, int slots,
                       uint64_t budget_bytes);
 int rfx_server_destroy(rfx_server* srv);
-/* Register resident relations for one plan; *schema_id names them. */
+/* Register resident relations for one plan; *schema_id names them.  The
+ * relations must stay resident and unchanged while the server runs.  For
+ * the join plans the joins are run once here: every join may be
+ * many-to-many, and each request's buffers are sized from the measured
+ * join sizes (a request whose join makes another number of rows ends
+ * with RFX_EINVAL). */
 int rfx_server_add_schema(rfx_server* srv, int plan, const rfx_relation* rels,
                           int nrels, uint32_t* schema_id);
 /* Budget bytes a request needs (its whole plan, workspace included). */

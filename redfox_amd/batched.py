@@ -7,7 +7,9 @@ The query of each instance i is
 
 with dim_i = (dkey: a permutation of 0..d_i-1, grp: U[0, 64)) and fact_i =
 (fk: Zipf(1.1) rank over [1, d_i] mapped through dkey, measure: U{0..100000}
-/ 100).  Two executions of the same operator plan are provided:
+/ 100).  With dup > 0 the first floor(dup * d_i) keys of the permutation
+appear a second time (with their own grp), so the join is many-to-many.
+Two executions of the same operator plan are provided:
 
 * plain: the way Harmony runs a Red Fox plan - one instance after another,
   one operator launch at a time, the host reading each size variable before
@@ -37,19 +39,23 @@ class Workload:
     device.  fact_exp / dim_exp give the log10 ranges of the row counts."""
 
     def __init__(self, ctx: R.Context, n_inst: int = 512, seed: int = 0x52464F58000000CF,
-                 fact_exp=(3.0, 7.0), dim_exp=(2.0, 5.0), zipf_s: float = 1.1):
+                 fact_exp=(3.0, 7.0), dim_exp=(2.0, 5.0), zipf_s: float = 1.1, dup: float = 0.0):
         self.ctx = ctx
         self.n_inst = n_inst
         u = uniform01(seed, 2 * n_inst)
         self.n = np.floor(10.0 ** (fact_exp[0] + (fact_exp[1] - fact_exp[0]) * u[:n_inst])).astype(np.int64)
-        self.d = np.floor(10.0 ** (dim_exp[0] + (dim_exp[1] - dim_exp[0]) * u[n_inst:])).astype(np.int64)
-        assert int(self.d.max()) <= (1 << FK_BITS)
+        keys = np.floor(10.0 ** (dim_exp[0] + (dim_exp[1] - dim_exp[0]) * u[n_inst:])).astype(np.int64)
+        assert int(keys.max()) <= (1 << FK_BITS)
+        extra = np.floor(dup * keys).astype(np.int64)
+        self.d = keys + extra  # dimension rows
         self.row_off = np.concatenate([[0], np.cumsum(self.n)]).astype(np.uint64)
         self.dim_off = np.concatenate([[0], np.cumsum(self.d)]).astype(np.uint64)
         self.N = int(self.row_off[-1])
         self.D = int(self.dim_off[-1])
         rng = np.random.default_rng(seed & 0xFFFFFFFF)
-        self.dkey_host = np.concatenate([rng.permutation(int(x)) for x in self.d]).astype(np.uint32)
+        perms = [rng.permutation(int(x)) for x in keys]
+        self.dkey_host = np.concatenate(
+            [np.concatenate([p, p[:int(e)]]) for p, e in zip(perms, extra)]).astype(np.uint32)
         self.grp_host = rng.integers(0, GROUPS, self.D, dtype=np.uint32)
         self.dkey = R.DeviceBuffer.from_numpy(self.dkey_host)
         self.grp = R.DeviceBuffer.from_numpy(self.grp_host)
@@ -87,14 +93,24 @@ class PlainRunner:
         self.dkw, self.dgw = B(4 * dmax), B(4 * dmax)
         self.lb, self.lc = B(4 * nmax), B(4 * nmax)
         self.tot = R.u64_cell()
-        self.out16 = B(16 * nmax)
-        self.key2, self.val = B(4 * nmax), B(8 * nmax)
+        self.jcap = 0
+        self._resize(nmax)
         self.okeys = B(4 * GROUPS * w.n_inst)
         self.osum = B(8 * GROUPS * w.n_inst)
         self.ocnt = B(4 * GROUPS * w.n_inst)
         self.okeys2 = B(4 * GROUPS)
         self.nout = R.DeviceBuffer(8 * w.n_inst)
         self.nout2 = R.u64_cell()
+
+    def _resize(self, rows: int) -> None:
+        """Joined-row buffers of at least `rows` rows (Harmony resizes a
+        variable to the size it read back)."""
+        if rows <= self.jcap:
+            return
+        B = R.DeviceBuffer
+        self.jcap = rows
+        self.out16 = B(16 * rows)
+        self.key2, self.val = B(4 * rows), B(8 * rows)
 
     def run(self) -> None:
         w, c = self.w, self.ctx
@@ -114,6 +130,7 @@ class PlainRunner:
             c.join_find_bounds(self.lb, self.lc, self.tot, self.fkw, n, self.dkw, d, 4)
             c.synchronize()
             t = R.read_u64(self.tot)  # join_get_result_size + Resize
+            self._resize(t)
             c.join_expand_gather(self.out16, 16, self.fkw, 4, self.mw, 8, self.dgw, 4, self.lb,
                                  self.lc, n, self.tot, t, 96, 32)
             c.synchronize()
@@ -166,14 +183,19 @@ class BatchedRunner:
         self.dck, self.dgw = B(4 * D), B(4 * D)
         self.lb, self.lc = B(4 * N), B(4 * N)
         self.tot = R.u64_cell()
-        self.key2, self.val = B(4 * N), B(8 * N)
         cap = GROUPS * w.n_inst
         self.okeys, self.osum, self.ocnt = B(4 * cap), B(8 * cap), B(4 * cap)
         self.nout = R.u64_cell()
         self.seg_bits = max(1, int(np.ceil(np.log2(w.n_inst))))
         self.graph = None
+        # The join's size, measured once: the plan's joined columns are
+        # sized from it (the relations are resident and do not change).
+        self._enqueue_bounds()
+        self.ctx.synchronize()
+        self.J = R.read_u64(self.tot)
+        self.key2, self.val = B(4 * max(self.J, 1)), B(8 * max(self.J, 1))
 
-    def _enqueue(self) -> None:
+    def _enqueue_bounds(self) -> None:
         w, c, N, D = self.w, self.ctx, self.w.N, self.w.D
         kb = FK_BITS + self.seg_bits
         c.segment_keys(self.ck, w.fk, w.row_off_dev, w.n_inst, N, FK_BITS)
@@ -183,17 +205,20 @@ class BatchedRunner:
         c.sort_pairs(self.ck, self.mw, N, 4, kb, 8)
         c.sort_pairs(self.dck, self.dgw, D, 4, kb, 4)
         c.join_find_bounds(self.lb, self.lc, self.tot, self.ck, N, self.dck, D, 4)
-        # Every fact key references its dimension, so the join has N rows.
+
+    def _enqueue(self) -> None:
+        w, c, J = self.w, self.ctx, self.J
+        self._enqueue_bounds()
         # The joined row goes straight to the (instance, grp) key column and
         # the measure column: gather + project + split in one pass.
-        c.join_expand_project(self.lb, self.lc, N, self.tot, N, self.ck, 4, self.mw, 8, self.dgw,
+        c.join_expand_project(self.lb, self.lc, w.N, self.tot, J, self.ck, 4, self.mw, 8, self.dgw,
                               4, self.key2, 4, self.val, 8,
                               [(0, 0, FK_BITS, self.seg_bits, GRP_BITS),  # instance
                                (0, 2, 0, GRP_BITS, 0),                    # grp
                                (1, 1, 0, 64, 0)])                         # measure
-        c.sort_pairs(self.key2, self.val, N, 4, self.seg_bits + GRP_BITS, 8)
+        c.sort_pairs(self.key2, self.val, J, 4, self.seg_bits + GRP_BITS, 8)
         c.reduce_count_by_key_f64(self.okeys, self.osum, self.ocnt, self.nout, self.key2, self.val,
-                                  self.tot, N, 4, 0, out_cap=GROUPS * w.n_inst)
+                                  self.tot, J, 4, 0, out_cap=GROUPS * w.n_inst)
 
     def autotune(self, reps: int = 2) -> dict:
         """Choose the operators' tile shapes for this workload by timing the
@@ -220,9 +245,9 @@ class BatchedRunner:
         self.ctx.synchronize()
 
     def results(self):
-        if R.read_u64(self.tot) != self.w.N:
-            raise RuntimeError("batched join: a fact key has no dimension row "
-                               "(the plan assumes referential integrity)")
+        if R.read_u64(self.tot) != self.J:
+            raise RuntimeError("batched join: the join's size changed since the plan was sized "
+                               "(the relations must stay unchanged)")
         g = R.read_u64(self.nout)
         keys = self.okeys.download(np.uint32, g)
         return _results(keys, self.osum.download(np.float64, g), self.ocnt.download(np.uint32, g),

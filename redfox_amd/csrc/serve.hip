@@ -33,7 +33,7 @@ namespace rfx {
 namespace {
 
 constexpr int kResWords = 8;  // rows, digest[3], flags, admit/done (host), spare
-constexpr uint64_t kFlagNot1to1 = 1;  // a join did not match every left row once
+constexpr uint64_t kFlagJoinSize = 1;  // a join's size differs from its registered size
 
 __device__ __forceinline__ uint64_t fmix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -168,6 +168,11 @@ struct Schema {
   int plan = 0;
   int nrel = 0;
   rfx_relation rel[3] = {};
+  // Rows each join of the plan produces (join_agg: [0]; join3: the first
+  // and the second join), measured once at registration: the relations
+  // stay resident and unchanged, so every request's join has these sizes,
+  // and its buffers are sized from them.
+  uint64_t join_rows[2] = {0, 0};
 };
 
 struct Slot {
@@ -285,7 +290,7 @@ class Plan {
     return step(RFX_OK);
   }
   int check_total(const uint64_t* total, uint64_t expect) {
-    k_check_total<<<1, 64, 0, ctx_->stream>>>(res_, total, expect, kFlagNot1to1);
+    k_check_total<<<1, 64, 0, ctx_->stream>>>(res_, total, expect, kFlagJoinSize);
     RFX_LAUNCHED();
     return RFX_OK;
   }
@@ -316,13 +321,15 @@ class Plan {
   }
 
   // SELECT grp, SUM(m), COUNT(*) FROM F JOIN D ON fk = dkey GROUP BY grp:
-  // F (fk u32, m f64), D (dkey u32 unique, grp u32 < 2^grp_bits).
+  // F (fk u32, m f64), D (dkey u32, grp u32 < 2^grp_bits); any key may
+  // repeat on either side.
   int join_agg(uint64_t* need) {
     const rfx_relation& f = sc_.rel[0];
     const rfx_relation& d = sc_.rel[1];
     const uint64_t nf = f.rows, nd = d.rows;
     const uint32_t gbits = d.val_bits;  // bits of grp
     const uint64_t groups = 1ull << gbits;
+    const uint64_t nj = sc_.join_rows[0];  // joined rows
     uint32_t* fk = c_.take<uint32_t>(nf);
     double* m = c_.take<double>(nf);
     uint32_t* dk = c_.take<uint32_t>(nd);
@@ -330,17 +337,18 @@ class Plan {
     int32_t* lb = c_.take<int32_t>(nf);
     int32_t* lc = c_.take<int32_t>(nf);
     uint64_t* tot = c_.take<uint64_t>(1);
-    uint32_t* key2 = c_.take<uint32_t>(nf);
-    double* val = c_.take<double>(nf);
+    uint32_t* key2 = c_.take<uint32_t>(nj ? nj : 1);
+    double* val = c_.take<double>(nj ? nj : 1);
     uint32_t* ok = c_.take<uint32_t>(groups);
     double* osum = c_.take<double>(groups);
     uint32_t* ok2 = c_.take<uint32_t>(groups);
     uint32_t* ocnt = c_.take<uint32_t>(groups);
     uint64_t* nout = c_.take<uint64_t>(1);
     uint64_t* nout2 = c_.take<uint64_t>(1);
-    const uint64_t ws = std::max<uint64_t>(
+    uint64_t ws = std::max<uint64_t>(
         std::max(rfx_sort_workspace(nf, 4, f.key_bits, 8), rfx_sort_workspace(nd, 4, d.key_bits, 4)),
-        small_ws(2 * nf + nd));
+        small_ws(2 * nf + nd + nj));
+    ws = std::max(ws, rfx_sort_workspace(nj, 4, static_cast<int>(gbits), 8));
     if (layout_done(ws, need)) return RFX_OK;
     RFX_TRY(begin());
     RFX_TRY(copy(fk, f.col[0], 4 * nf));
@@ -350,9 +358,9 @@ class Plan {
     RFX_TRY(step(rfx_sort_pairs(ctx_, fk, m, nf, 4, f.key_bits, 8)));
     RFX_TRY(step(rfx_sort_pairs(ctx_, dk, dg, nd, 4, d.key_bits, 4)));
     RFX_TRY(step(rfx_join_find_bounds(ctx_, lb, lc, tot, fk, nf, dk, nd, 4)));
-    uint64_t t = nf;
-    if (plain_) RFX_TRY(read_capped(tot, nf, &t));
-    RFX_TRY(check_total(tot, nf));
+    uint64_t t = nj;
+    if (plain_) RFX_TRY(read_capped(tot, nj, &t));
+    RFX_TRY(check_total(tot, nj));
     const uint32_t fields[10] = {0, 2, 0, gbits, 0,   // grp -> key
                                  1, 1, 0, 64, 0};     // m -> value
     RFX_TRY(step(rfx_join_expand_project(ctx_, lb, lc, nf, tot, t, fk, 4, m, 8, dg, 4, key2, 4,
@@ -365,9 +373,9 @@ class Plan {
       RFX_TRY(step(rfx_count_by_key(ctx_, ok2, ocnt, nout2, key2, tot, t, 4)));
       RFX_TRY(read(nout2, &g));
     } else {  // both aggregates from one reduce-by-key (the same bits)
-      // The outputs hold `groups` rows: a join that broke the 1:1 rule
-      // (flagged above) leaves unwritten rows in key2 that may form more
-      // runs; those are counted but never written.
+      // The outputs hold `groups` rows; any run past them (possible only
+      // when the join's size differs from the registered one, flagged
+      // above) is counted but never written.
       RFX_TRY(step(rfx_reduce_count_by_key_f64(ctx_, ok, osum, ocnt, groups, nout, key2, val, tot,
                                                t, 4, RFX_AGG_SUM)));
     }
@@ -378,13 +386,14 @@ class Plan {
   }
 
   // Chain F ⋈ S ⋈ T on F.fk1 = S.skey and S.fk2 = T.tkey, output (T.tval,
-  // F.m) in join order: F (fk1 u32, m f64), S (skey u32 unique, fk2 u32),
-  // T (tkey u32 unique, tval u32).  S.val_bits bounds fk2.
+  // F.m) in join order: F (fk1 u32, m f64), S (skey u32, fk2 u32), T (tkey
+  // u32, tval u32), keys repeating on either side.  S.val_bits bounds fk2.
   int join3(uint64_t* need) {
     const rfx_relation& f = sc_.rel[0];
     const rfx_relation& s = sc_.rel[1];
     const rfx_relation& t = sc_.rel[2];
     const uint64_t nf = f.rows, ns = s.rows, nt = t.rows;
+    const uint64_t j1 = sc_.join_rows[0], j2 = sc_.join_rows[1];  // rows of each join
     uint32_t* f1 = c_.take<uint32_t>(nf);
     double* fm = c_.take<double>(nf);
     uint32_t* sk = c_.take<uint32_t>(ns);
@@ -392,18 +401,20 @@ class Plan {
     int32_t* lb = c_.take<int32_t>(nf);
     int32_t* lc = c_.take<int32_t>(nf);
     uint64_t* tot = c_.take<uint64_t>(1);
-    uint32_t* k2 = c_.take<uint32_t>(nf);
-    double* v2 = c_.take<double>(nf);
+    uint32_t* k2 = c_.take<uint32_t>(j1 ? j1 : 1);
+    double* v2 = c_.take<double>(j1 ? j1 : 1);
     uint32_t* tk = c_.take<uint32_t>(nt);
     uint32_t* tv = c_.take<uint32_t>(nt);
+    int32_t* lb2 = c_.take<int32_t>(j1 ? j1 : 1);
+    int32_t* lc2 = c_.take<int32_t>(j1 ? j1 : 1);
     uint64_t* tot2 = c_.take<uint64_t>(1);
-    uint32_t* ok = c_.take<uint32_t>(nf);
-    double* ov = c_.take<double>(nf);
+    uint32_t* ok = c_.take<uint32_t>(j2 ? j2 : 1);
+    double* ov = c_.take<double>(j2 ? j2 : 1);
     uint64_t ws = std::max(rfx_sort_workspace(nf, 4, f.key_bits, 8),
-                           rfx_sort_workspace(nf, 4, s.val_bits, 8));
+                           rfx_sort_workspace(j1, 4, s.val_bits, 8));
     ws = std::max(ws, rfx_sort_workspace(ns, 4, s.key_bits, 4));
     ws = std::max(ws, rfx_sort_workspace(nt, 4, t.key_bits, 4));
-    ws = std::max(ws, small_ws(2 * nf + ns + nt));
+    ws = std::max(ws, small_ws(nf + j1 + j2 + ns + nt));
     if (layout_done(ws, need)) return RFX_OK;
     RFX_TRY(begin());
     RFX_TRY(copy(f1, f.col[0], 4 * nf));
@@ -413,9 +424,9 @@ class Plan {
     RFX_TRY(step(rfx_sort_pairs(ctx_, f1, fm, nf, 4, f.key_bits, 8)));
     RFX_TRY(step(rfx_sort_pairs(ctx_, sk, sf, ns, 4, s.key_bits, 4)));
     RFX_TRY(step(rfx_join_find_bounds(ctx_, lb, lc, tot, f1, nf, sk, ns, 4)));
-    uint64_t n1 = nf;
-    if (plain_) RFX_TRY(read_capped(tot, nf, &n1));
-    RFX_TRY(check_total(tot, nf));
+    uint64_t n1 = j1;
+    if (plain_) RFX_TRY(read_capped(tot, j1, &n1));
+    RFX_TRY(check_total(tot, j1));
     const uint32_t fields1[10] = {0, 2, 0, 32, 0,   // S.fk2 -> key
                                   1, 1, 0, 64, 0};  // F.m -> value
     RFX_TRY(step(rfx_join_expand_project(ctx_, lb, lc, nf, tot, n1, f1, 4, fm, 8, sf, 4, k2, 4,
@@ -424,18 +435,17 @@ class Plan {
     RFX_TRY(copy(tk, t.col[0], 4 * nt));
     RFX_TRY(copy(tv, t.col[1], 4 * nt));
     RFX_TRY(step(rfx_sort_pairs(ctx_, tk, tv, nt, 4, t.key_bits, 4)));
-    // The second join reuses lb/lc (the first expansion has consumed them).
-    RFX_TRY(step(rfx_join_find_bounds(ctx_, lb, lc, tot2, k2, n1, tk, nt, 4)));
-    uint64_t n2 = nf;
-    if (plain_) RFX_TRY(read_capped(tot2, nf, &n2));
-    RFX_TRY(check_total(tot2, nf));
+    RFX_TRY(step(rfx_join_find_bounds(ctx_, lb2, lc2, tot2, k2, n1, tk, nt, 4)));
+    uint64_t n2 = j2;
+    if (plain_) RFX_TRY(read_capped(tot2, j2, &n2));
+    RFX_TRY(check_total(tot2, j2));
     const uint32_t fields2[10] = {0, 2, 0, 32, 0,   // T.tval -> key
                                   1, 1, 0, 64, 0};  // m -> value
-    RFX_TRY(step(rfx_join_expand_project(ctx_, lb, lc, n1, tot2, n2, k2, 4, v2, 8, tv, 4, ok, 4,
+    RFX_TRY(step(rfx_join_expand_project(ctx_, lb2, lc2, n1, tot2, n2, k2, 4, v2, 8, tv, 4, ok, 4,
                                          ov, 8, 2, fields2)));
-    RFX_TRY(finish(tot2, nf));
-    RFX_TRY(digest_col<4>(0, ok, nf));
-    return digest_col<8>(1, ov, nf);
+    RFX_TRY(finish(tot2, j2));
+    RFX_TRY(digest_col<4>(0, ok, j2));
+    return digest_col<8>(1, ov, j2);
   }
 
   // A op B on sorted unique u64 sets (unsigned order); param = RFX_SET_*.
@@ -475,6 +485,87 @@ int validate(const rfx_server* s, const rfx_request& rq) {
     return fail(RFX_EINVAL, "serve: set operation %u", rq.param);
   if (sc.plan == RFX_PLAN_SELECT_PROJECT && rq.param > RFX_CMP_SGE)
     return fail(RFX_EINVAL, "serve: comparison %u", rq.param);
+  return RFX_OK;
+}
+
+// Device memory for one registration, freed on every path.
+struct TempMem {
+  std::vector<void*> p;
+  ~TempMem() {
+    for (void* q : p) (void)hipFree(q);
+  }
+  template <class T>
+  int take(uint64_t n, T** out) {
+    void* q = nullptr;
+    if (hipMalloc(&q, (n ? n : 1) * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(RFX_ENOMEM, "serve: %llu bytes to measure a join",
+                  static_cast<unsigned long long>(n * sizeof(T)));
+    }
+    p.push_back(q);
+    *out = static_cast<T*>(q);
+    return RFX_OK;
+  }
+};
+
+// Rows of each join of a join_agg or join3 plan over the schema's
+// relations (the same sorts and find_bounds the plan runs, once, on a
+// context of its own; for join3 the second join's left keys are S.fk2 of
+// the first join's rows).
+int measure_joins(int device, Schema* sc) {
+  const rfx_relation& f = sc->rel[0];
+  const rfx_relation& d = sc->rel[1];
+  rfx_ctx* ctx = nullptr;
+  RFX_TRY(rfx_ctx_create(&ctx, device, nullptr));
+  struct Guard {
+    rfx_ctx* c;
+    ~Guard() { (void)rfx_ctx_destroy(c); }
+  } guard{ctx};
+  TempMem mem;
+  uint32_t *fk, *dk, *dv;
+  int32_t *lb, *lc;
+  uint64_t* tot;
+  RFX_TRY(mem.take(f.rows, &fk));
+  RFX_TRY(mem.take(d.rows, &dk));
+  RFX_TRY(mem.take(d.rows, &dv));
+  RFX_TRY(mem.take(f.rows, &lb));
+  RFX_TRY(mem.take(f.rows, &lc));
+  RFX_TRY(mem.take(2, &tot));
+  RFX_TRY(rfx_copy(ctx, fk, f.col[0], 4 * f.rows));
+  RFX_TRY(rfx_copy(ctx, dk, d.col[0], 4 * d.rows));
+  RFX_TRY(rfx_copy(ctx, dv, d.col[1], 4 * d.rows));
+  RFX_TRY(rfx_sort_pairs(ctx, fk, nullptr, f.rows, 4, static_cast<int>(f.key_bits), 0));
+  RFX_TRY(rfx_sort_pairs(ctx, dk, dv, d.rows, 4, static_cast<int>(d.key_bits), 4));
+  RFX_TRY(rfx_join_find_bounds(ctx, lb, lc, tot, fk, f.rows, dk, d.rows, 4));
+  RFX_TRY(rfx_ctx_synchronize(ctx));
+  uint64_t t1 = 0;
+  RFX_HIP(hipMemcpy(&t1, tot, 8, hipMemcpyDeviceToHost));
+  sc->join_rows[0] = t1;
+  if (sc->plan != RFX_PLAN_JOIN3) return RFX_OK;
+  if (t1 >= (1ull << 30))
+    return fail(RFX_ERANGE, "serve: the first join makes %llu rows (max 2^30-1)",
+                static_cast<unsigned long long>(t1));
+  const rfx_relation& t = sc->rel[2];
+  uint32_t *k2, *tk;
+  int32_t *lb2, *lc2;
+  RFX_TRY(mem.take(t1, &k2));
+  RFX_TRY(mem.take(t.rows, &tk));
+  RFX_TRY(mem.take(t1, &lb2));
+  RFX_TRY(mem.take(t1, &lc2));
+  const uint32_t field[5] = {0, 2, 0, 32, 0};  // S.fk2 of each joined row
+  RFX_TRY(rfx_join_expand_project(ctx, lb, lc, f.rows, tot, t1, fk, 4, nullptr, 0, dv, 4, k2, 4,
+                                  nullptr, 0, 1, field));
+  RFX_TRY(rfx_sort_pairs(ctx, k2, nullptr, t1, 4, static_cast<int>(d.val_bits), 0));
+  RFX_TRY(rfx_copy(ctx, tk, t.col[0], 4 * t.rows));
+  RFX_TRY(rfx_sort_pairs(ctx, tk, nullptr, t.rows, 4, static_cast<int>(t.key_bits), 0));
+  RFX_TRY(rfx_join_find_bounds(ctx, lb2, lc2, tot + 1, k2, t1, tk, t.rows, 4));
+  RFX_TRY(rfx_ctx_synchronize(ctx));
+  uint64_t t2 = 0;
+  RFX_HIP(hipMemcpy(&t2, tot + 1, 8, hipMemcpyDeviceToHost));
+  if (t2 >= (1ull << 30))
+    return fail(RFX_ERANGE, "serve: the second join makes %llu rows (max 2^30-1)",
+                static_cast<unsigned long long>(t2));
+  sc->join_rows[1] = t2;
   return RFX_OK;
 }
 
@@ -576,6 +667,8 @@ int rfx_server_add_schema(rfx_server* s, int plan, const rfx_relation* rels, int
   if (plan == RFX_PLAN_JOIN_AGG && (rels[1].val_bits < 1 || rels[1].val_bits > 16))
     return fail(RFX_EINVAL, "rfx_server_add_schema: group column of %u bits (1..16)",
                 rels[1].val_bits);
+  if (plan == RFX_PLAN_JOIN_AGG || plan == RFX_PLAN_JOIN3)
+    RFX_TRY(measure_joins(s->device, &sc));
   *schema_id = static_cast<uint32_t>(s->schemas.size());
   s->schemas.push_back(sc);
   return RFX_OK;
@@ -680,7 +773,7 @@ int rfx_server_run(rfx_server* s, rfx_request* reqs, uint64_t n, int mode, rfx_s
     const uint64_t* r = &host[i * kResWords];
     reqs[i].rows = r[0];
     for (int c = 0; c < 3; ++c) reqs[i].digest[c] = r[1 + c];
-    if (r[4] & kFlagNot1to1) reqs[i].status = RFX_EINVAL;
+    if (r[4] & kFlagJoinSize) reqs[i].status = RFX_EINVAL;
   }
   if (st) {
     st->wall_ns = wall;

@@ -10,18 +10,28 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def expected(w, fk, measure):
+def expected(w, fk, measure, which=None):
+    """Per instance: the sort-merge join's row pairs (each fact row with
+    every dimension row of its key), then the groups' fsum and count."""
     out = []
-    for b in range(w.n_inst):
+    for b in range(w.n_inst) if which is None else which:
         r0, r1 = int(w.row_off[b]), int(w.row_off[b + 1])
         d0, d1 = int(w.dim_off[b]), int(w.dim_off[b + 1])
-        grp_of = np.empty(d1 - d0, dtype=np.uint32)
-        grp_of[w.dkey_host[d0:d1]] = w.grp_host[d0:d1]
-        g = grp_of[fk[r0:r1]]
-        keys = np.unique(g)
-        sums = np.array([math.fsum(measure[r0:r1][g == k].tolist()) for k in keys])
-        counts = np.array([(g == k).sum() for k in keys], dtype=np.uint32)
-        out.append((keys, sums, counts))
+        lk, rk = fk[r0:r1], w.dkey_host[d0:d1]
+        ro = np.argsort(rk, kind="stable")
+        rks = rk[ro]
+        lo = np.searchsorted(rks, lk, "left")
+        cnt = np.searchsorted(rks, lk, "right") - lo
+        li = np.repeat(np.arange(len(lk)), cnt)
+        ri = lo[li] + (np.arange(len(li)) - (np.cumsum(cnt) - cnt)[li])
+        g = w.grp_host[d0:d1][ro][ri]
+        m = measure[r0:r1][li]
+        o = np.argsort(g, kind="stable")
+        keys, first, counts = np.unique(g[o], return_index=True, return_counts=True)
+        ms = m[o]
+        ends = np.append(first[1:], len(o))
+        sums = np.array([math.fsum(ms[a:e].tolist()) for a, e in zip(first, ends)])
+        out.append((keys, sums, counts.astype(np.uint32)))
     return out
 
 
@@ -33,10 +43,14 @@ def check(a, b):
         np.testing.assert_allclose(sa, sb, rtol=1e-12, atol=0)
 
 
-def test_batched_equals_plain_equals_numpy(rf, ctx):
+@pytest.mark.parametrize("dup", [0.0, 0.4])
+def test_batched_equals_plain_equals_numpy(rf, ctx, dup):
+    """dup = 0: every fact row joins one dimension row; dup = 0.4: 40% of
+    each dimension's keys appear twice, so the join is many-to-many."""
     from redfox_amd import batched
 
-    w = batched.Workload(ctx, n_inst=48, seed=11, fact_exp=(2.0, 5.0), dim_exp=(1.0, 3.5))
+    w = batched.Workload(ctx, n_inst=48, seed=11, fact_exp=(2.0, 5.0), dim_exp=(1.0, 3.5),
+                         dup=dup)
     fk = w.fk.download(np.uint32, w.N)
     measure = w.measure.download(np.float64, w.N)
     # every fk references a dimension row
@@ -48,6 +62,7 @@ def test_batched_equals_plain_equals_numpy(rf, ctx):
     plain.run()
     check(plain.results(), want)
     br = batched.BatchedRunner(w)
+    assert (br.J == w.N) == (dup == 0.0)
     br.capture()
     for _ in range(3):  # replays give the same results
         br.run()
@@ -70,3 +85,23 @@ def test_zipf_foreign_keys_are_skewed(rf, ctx):
     assert top[0] / top[1] > 1.5
     # the hottest key is the dimension's first key (rank 1 through dkey)
     assert int(np.argmax(counts)) == int(w.dkey_host[0])
+
+
+def test_batched_512_instances_sample_against_numpy(rf, ctx):
+    """The measured workload (512 instances, 5.6e8 fact rows): the graph
+    replay's results for the largest and smallest instance and six more
+    chosen at random equal numpy's."""
+    from redfox_amd import batched
+
+    w = batched.Workload(ctx, n_inst=512)
+    br = batched.BatchedRunner(w)
+    br.capture()
+    br.run()
+    br.synchronize()
+    got = br.results()
+    rng = np.random.default_rng(512)
+    which = sorted({int(np.argmax(w.n)), int(np.argmin(w.n))} |
+                   set(int(x) for x in rng.choice(w.n_inst, 6, replace=False)))
+    fk = w.fk.download(np.uint32, w.N)
+    measure = w.measure.download(np.float64, w.N)
+    check([got[b] for b in which], expected(w, fk, measure, which))

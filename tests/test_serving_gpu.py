@@ -71,34 +71,51 @@ def expected_select_project(sch, constant):
     return [((hi[keep] >> np.uint64(32)) << np.uint64(32)) | (lo[keep] & np.uint64(0xFFFFFFFF))]
 
 
+def join_pairs(lk, rk):
+    """Row pairs of a sort-merge join of sorted key arrays, in the order the
+    expansion writes them: left rows in order, each left row's matches in
+    right order."""
+    lo = np.searchsorted(rk, lk, "left")
+    cnt = np.searchsorted(rk, lk, "right") - lo
+    li = np.repeat(np.arange(len(lk)), cnt)
+    start = np.cumsum(cnt) - cnt
+    ri = lo[li] + (np.arange(len(li)) - start[li])
+    return li, ri
+
+
+def expected_join_agg_cols(fk, m, dk, g):
+    of, od = np.argsort(fk, kind="stable"), np.argsort(dk, kind="stable")
+    li, ri = join_pairs(fk[of], dk[od])
+    gf, mf = g[od][ri], m[of][li]
+    keys = np.unique(gf)
+    sums = np.array([math.fsum(mf[gf == k].tolist()) for k in keys])
+    counts = np.array([(gf == k).sum() for k in keys], dtype=np.uint32)
+    return [keys, sums, counts]
+
+
 def expected_join_agg(sch):
     fk_b, m_b, dk_b, g_b = sch["bufs"]
     nf, nd = sch["sizes"]
-    fk, m = fk_b.download(np.uint32, nf), m_b.download(np.float64, nf)
-    dk, g = dk_b.download(np.uint32, nd), g_b.download(np.uint32, nd)
-    grp_of = np.empty(nd, dtype=np.uint32)
-    grp_of[dk] = g
-    gf = grp_of[fk]
-    keys = np.unique(gf)
-    sums = np.array([math.fsum(m[gf == k].tolist()) for k in keys])
-    counts = np.array([(gf == k).sum() for k in keys], dtype=np.uint32)
-    return [keys, sums, counts]
+    return expected_join_agg_cols(fk_b.download(np.uint32, nf), m_b.download(np.float64, nf),
+                                  dk_b.download(np.uint32, nd), g_b.download(np.uint32, nd))
+
+
+def expected_join3_cols(fk1, m, sk, fk2, tk, tv):
+    o1, os_, ot = (np.argsort(x, kind="stable") for x in (fk1, sk, tk))
+    li, ri = join_pairs(fk1[o1], sk[os_])
+    k2, v2 = fk2[os_][ri], m[o1][li]
+    o2 = np.argsort(k2, kind="stable")
+    k2, v2 = k2[o2], v2[o2]
+    li, ri = join_pairs(k2, tk[ot])
+    return [tv[ot][ri], v2[li]]
 
 
 def expected_join3(sch):
     fk1_b, m_b, sk_b, fk2_b, tk_b, tv_b = sch["bufs"]
     nf, ns, nt = sch["sizes"]
-    fk1, m = fk1_b.download(np.uint32, nf), m_b.download(np.float64, nf)
-    sk, fk2 = sk_b.download(np.uint32, ns), fk2_b.download(np.uint32, ns)
-    tk, tv = tk_b.download(np.uint32, nt), tv_b.download(np.uint32, nt)
-    fk2_of = np.empty(ns, dtype=np.uint32)
-    fk2_of[sk] = fk2
-    tv_of = np.empty(nt, dtype=np.uint32)
-    tv_of[tk] = tv
-    o1 = np.argsort(fk1, kind="stable")
-    k2, v2 = fk2_of[fk1[o1]], m[o1]
-    o2 = np.argsort(k2, kind="stable")
-    return [tv_of[k2[o2]], v2[o2]]
+    return expected_join3_cols(fk1_b.download(np.uint32, nf), m_b.download(np.float64, nf),
+                               sk_b.download(np.uint32, ns), fk2_b.download(np.uint32, ns),
+                               tk_b.download(np.uint32, nt), tv_b.download(np.uint32, nt))
 
 
 def expected_set(sch, op):
@@ -178,19 +195,71 @@ def test_request_errors(served, rf):
     tiny.close()
 
 
-def test_join_that_is_not_one_to_one_is_flagged(served, rf, ctx):
+def check_sink(sink, rows, exp, widths, cap, rtol):
+    off = 0
+    for w, e in zip(widths, exp):
+        is_f64 = e.dtype == np.float64
+        got = sink.download(np.float64 if is_f64 else {4: np.uint32, 8: np.uint64}[w], rows,
+                            offset=off)
+        if is_f64 and rtol:
+            np.testing.assert_allclose(got, e, rtol=rtol, atol=0)
+        else:
+            assert np.array_equal(got, e)
+        off += cap * w
+
+
+def test_many_to_many_joins_match_numpy(served, rf, ctx):
+    """Dimension keys that repeat (a quarter of them twice) or are missing
+    (the most frequent Zipf rank): each fact row joins every matching row.
+    The join sizes are measured when the relations are registered, and
+    every form of the request gives the numpy result."""
     S, srv, cat = served
     sch = cat.schemas[1][0]
     nf, nd = sch["sizes"]
-    fk, m, dk, g = sch["bufs"]
-    # Drop the first dimension row (the most frequent Zipf rank): the fact
-    # rows referencing it find no match.
-    rels = [sch["rels"][0]]
+    fk_b, m_b, dk_b, g_b = sch["bufs"]
+    fk, m = fk_b.download(np.uint32, nf), m_b.download(np.float64, nf)
+    dk, g = dk_b.download(np.uint32, nd), g_b.download(np.uint32, nd)
+    q = nd // 4
+    dk2, g2 = np.concatenate([dk[1:], dk[:q]]), np.concatenate([g[1:], g[:q]])
+    bd, bg = rf.DeviceBuffer.from_numpy(dk2), rf.DeviceBuffer.from_numpy(g2)
     r1 = S.Relation()
-    r1.col[0], r1.col[1] = dk.ptr + 4, g.ptr + 4
-    r1.rows, r1.key_bits, r1.val_bits = nd - 1, sch["rels"][1].key_bits, sch["rels"][1].val_bits
-    sid = srv.add_schema(1, [rels[0], r1])
-    req = (S.Request * 1)()
-    req[0].schema = sid
-    srv.run(req)
-    assert req[0].status != 0
+    r1.col[0], r1.col[1] = bd.ptr, bg.ptr
+    r1.rows, r1.key_bits, r1.val_bits = len(dk2), sch["rels"][1].key_bits, sch["rels"][1].val_bits
+    sid_a = srv.add_schema(1, [sch["rels"][0], r1])
+    exp_a = expected_join_agg_cols(fk, m, dk2, g2)
+
+    s3 = cat.schemas[2][0]
+    nf3, ns, nt = s3["sizes"]
+    fk1_b, m3_b, sk_b, fk2_b, tk_b, tv_b = s3["bufs"]
+    sk, fk2 = sk_b.download(np.uint32, ns), fk2_b.download(np.uint32, ns)
+    tk, tv = tk_b.download(np.uint32, nt), tv_b.download(np.uint32, nt)
+    qs, qt = ns // 3, nt // 5
+    sk2, fk22 = np.concatenate([sk[1:], sk[:qs]]), np.concatenate([fk2[1:], fk2[:qs]])
+    tk2, tv2 = np.concatenate([tk, tk[:qt]]), np.concatenate([tv, tv[:qt] ^ np.uint32(0xA5A5)])
+    bufs3 = [rf.DeviceBuffer.from_numpy(x) for x in (sk2, fk22, tk2, tv2)]
+    rs, rt = S.Relation(), S.Relation()
+    rs.col[0], rs.col[1] = bufs3[0].ptr, bufs3[1].ptr
+    rs.rows, rs.key_bits, rs.val_bits = len(sk2), s3["rels"][1].key_bits, s3["rels"][1].val_bits
+    rt.col[0], rt.col[1] = bufs3[2].ptr, bufs3[3].ptr
+    rt.rows, rt.key_bits = len(tk2), s3["rels"][2].key_bits
+    sid_3 = srv.add_schema(2, [s3["rels"][0], rs, rt])
+    exp_3 = expected_join3_cols(fk1_b.download(np.uint32, nf3), m3_b.download(np.float64, nf3),
+                                sk2, fk22, tk2, tv2)
+    assert len(exp_3[0]) != nf3  # not a 1:1 chain
+
+    widths_a, widths_3 = [4, 8, 4], [4, 8]
+    cap_a, cap_3 = 64, len(exp_3[0])
+    for mode in (S.MODE_CONTINUOUS, S.MODE_PLAIN):
+        sinks = [rf.DeviceBuffer(cap_a * 16), rf.DeviceBuffer(max(cap_3 * 12, 8))]
+        req = (S.Request * 2)()
+        for r, sid, sink in zip(req, (sid_a, sid_3), sinks):
+            r.schema, r.sink, r.sink_bytes = sid, sink.ptr, sink.nbytes
+        srv.run(req, mode)
+        assert req[0].status == 0 and req[1].status == 0, (req[0].status, req[1].status)
+        assert req[0].rows == len(exp_a[0]) and req[1].rows == cap_3
+        check_sink(sinks[0], req[0].rows, exp_a, widths_a, cap_a, 1e-12)
+        check_sink(sinks[1], req[1].rows, exp_3, widths_3, cap_3, 0)
+        for sink in sinks:
+            sink.close()
+    for b in [bd, bg] + bufs3:
+        b.close()

@@ -126,3 +126,30 @@ def test_full_size_u32_pairs_properties(rf, ctx):
     assert np.all(v[1:][same] > v[:-1][same])
     src = datagen.u32(seed, n)
     assert np.array_equal(src[v], k)
+
+
+def test_full_size_u64_zipf_pairs_properties(rf, ctx):
+    """2^27 (u64 Zipf(1.1) key, u32 row index) pairs, the size the sort is
+    measured at: sorted, stable and a permutation of the input (the keys
+    are generated on the device and copied before the sort)."""
+    n = 1 << 27
+    seed = datagen.BASE_SEED + 2
+    k0 = rf.DeviceBuffer(8 * n)
+    kbuf = rf.DeviceBuffer(8 * n)
+    vbuf = rf.DeviceBuffer(4 * n)
+    ctx.gen_zipf_u64(k0, n, seed, 1.1, 1 << 32)
+    ctx.copy(kbuf, k0, 8 * n)
+    ctx.iota_u32(vbuf, n)
+    ctx.sort_pairs(kbuf, vbuf, n, 8, 64, 4)
+    ctx.synchronize()
+    src = k0.download(np.uint64, n)
+    k = kbuf.download(np.uint64, n)
+    v = vbuf.download(np.uint32, n)
+    for b in (k0, kbuf, vbuf):
+        b.close()
+    assert np.all(k[1:] >= k[:-1])
+    same = k[1:] == k[:-1]
+    assert same.mean() > 0.3  # Zipf(1.1): many rows repeat a key
+    assert np.all(v[1:][same] > v[:-1][same])
+    assert np.array_equal(src[v], k)
+    assert np.array_equal(np.sort(v), np.arange(n, dtype=np.uint32))
