@@ -87,20 +87,31 @@ __global__ __launch_bounds__(256) void k_fb_partition(const K* __restrict__ a, u
 // (nearly) free of bank conflicts.
 __device__ __forceinline__ uint32_t cnt_slot(uint32_t i) { return i + (i >> 5); }
 
-// One tile of the merge path: its left keys A[a0, a1) and right keys
-// B[b0, b1).  With A winning ties, every B[j] before b0 is < A[a0] and every
-// B[j] from b1 on is >= A[a1 - 1], so each left key's lower bound lies in
-// [b0, b1].  The tile's right keys are followed in LDS by the next NT right
-// keys (B[b1, b1 + NT)), loaded with the rest, so the equal run of a left
-// key that crosses the tile's end - the last key of nearly every tile - is
-// measured in LDS too; only a run longer than that continues in global
-// memory (a gallop from b1 + NT).  Lower and upper bounds come from
-// fixed-step binary searches in LDS, four left keys per thread in lockstep
-// (neighbouring lanes probe the same words).  The counts are then scanned
-// in left-key order across the block and, by decoupled look-back, across
-// tiles.
+// Sub-tiles per block.  Blocks are ordered by an atomic ticket, so that the
+// decoupled look-back only ever waits on blocks already running.  Device-
+// scope atomics on one word complete one after another, though: with one
+// ticket per 4096-key tile they bounded the kernel (2^24 x 2^26 keys: 0.25
+// ms with the searches and the look-back taken out, 0.09 ms with the ticket
+// taken out as well).  A block of kFbSub x kFbNT threads therefore works on
+// kFbSub consecutive tiles at once, one per kFbNT-thread group, under one
+// ticket and one status word.
+constexpr int kFbSub = 4;
+
+// One tile of the merge path per thread group: its left keys A[a0, a1) and
+// right keys B[b0, b1).  With A winning ties, every B[j] before b0 is <
+// A[a0] and every B[j] from b1 on is >= A[a1 - 1], so each left key's lower
+// bound lies in [b0, b1].  The tile's right keys are followed in LDS by the
+// next NT right keys (B[b1, b1 + NT)), loaded with the rest, so the equal
+// run of a left key that crosses the tile's end - the last key of nearly
+// every tile - is measured in LDS too; only a run longer than that
+// continues in global memory (a gallop from b1 + NT).  Lower and upper
+// bounds come from fixed-step binary searches in LDS, four left keys per
+// thread in lockstep (neighbouring lanes probe the same words).  The counts
+// are then scanned in left-key order within the tile, the block's tiles are
+// added up in order, and the block takes its prefix from the blocks before
+// it by decoupled look-back.
 template <class K, int NT, int NV>
-__global__ __launch_bounds__(NT) void k_find_bounds(
+__global__ __launch_bounds__(NT * kFbSub, 8) void k_find_bounds(
     const K* __restrict__ a, uint64_t na, const K* __restrict__ b, uint64_t nb,
     const uint64_t* __restrict__ split, uint32_t ntiles, int32_t* __restrict__ lower_bound,
     int32_t* __restrict__ left_count, uint64_t* __restrict__ total,
@@ -108,49 +119,71 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
   constexpr int VT = NV / NT;
   static_assert(VT % 4 == 0, "left keys are searched four per thread at a time");
   constexpr int EXT = NT;
+  constexpr int NWS = NT / kWave;  // waves per tile group
   constexpr int KEY_BYTES = (NV + EXT) * static_cast<int>(sizeof(K));
   constexpr int CNT_BYTES = (NV + NV / 32) * 4;
+  constexpr int SUB_BYTES = ((KEY_BYTES > CNT_BYTES ? KEY_BYTES : CNT_BYTES) + 15) / 16 * 16;
   // The keys are dead once the searches are done; the counts reuse them.
-  __shared__ __attribute__((aligned(16))) unsigned char smem[KEY_BYTES > CNT_BYTES ? KEY_BYTES : CNT_BYTES];
-  __shared__ uint64_t scan_tmp[NT / kWave + 1];
+  __shared__ __attribute__((aligned(16))) unsigned char smem[kFbSub * SUB_BYTES];
+  __shared__ uint64_t s_wave[kFbSub * NWS];
+  __shared__ uint64_t s_sub[kFbSub];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
-  K* keys = reinterpret_cast<K*>(smem);
-  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(smem);
 
-  const uint32_t tid = threadIdx.x;
-  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
-  const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
-  const uint64_t d0 = static_cast<uint64_t>(tile) * NV;
-  const uint64_t d1 = (d0 + NV < na + nb) ? d0 + NV : na + nb;
-  const uint64_t a0 = split[tile], a1 = split[tile + 1];
-  const uint64_t b0 = d0 - a0, b1 = d1 - a1;
-  const uint32_t nat = static_cast<uint32_t>(a1 - a0), nbt = static_cast<uint32_t>(b1 - b0);
-  const uint32_t next = static_cast<uint32_t>(nb - b1 < EXT ? nb - b1 : EXT);
+  const uint32_t blk = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
+  const uint32_t sub = threadIdx.x / NT;                          // wave-uniform
+  const uint32_t tid = threadIdx.x % NT;
+  const uint32_t tile = blk * kFbSub + sub;
+  K* keys = reinterpret_cast<K*>(smem + sub * SUB_BYTES);
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(smem + sub * SUB_BYTES);
 
-  // The tile's A keys, its B keys, then the B keys past its end: every
-  // load in flight before the first LDS write.  Tile-relative 32-bit
-  // offsets from block-uniform bases keep the address arithmetic scalar.
+  // A group past the last tile holds an empty tile.
+  uint64_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+  if (tile < ntiles) {
+    const uint64_t d0 = static_cast<uint64_t>(tile) * NV;
+    const uint64_t d1 = (d0 + NV < na + nb) ? d0 + NV : na + nb;
+    a0 = split[tile];
+    a1 = split[tile + 1];
+    b0 = d0 - a0;
+    b1 = d1 - a1;
+  }
+  const uint32_t nat = static_cast<uint32_t>(a1 - a0), nbt = static_cast<uint32_t>(b1 - b0);
+  const uint32_t ntot = nat + nbt;
+  const uint32_t next = tile < ntiles ? static_cast<uint32_t>(nb - b1 < EXT ? nb - b1 : EXT) : 0u;
+
+  // The tile's A keys, its B keys, then the B keys past its end: every load
+  // in flight before the first LDS write.  Merged index i reads A[a0 + i]
+  // below nat and B[b0 + i - nat] from there: one address select per key,
+  // and no bounds test at all in a full tile.
   const K* __restrict__ at = a + a0;
   const K* __restrict__ bt = b + b0;
   const K* __restrict__ bx = b + b1;
   {
     K v[VT];
     K ev{};
+    if (ntot == NV) {
 #pragma unroll
-    for (int j = 0; j < VT; ++j) {
-      const uint32_t i = tid + j * NT;
-      if (i < nat) v[j] = at[i];
-      else if (i < nat + nbt) v[j] = bt[i - nat];
+      for (int j = 0; j < VT; ++j) {
+        const uint32_t i = tid + j * NT;
+        v[j] = *(i < nat ? at + i : bt + (i - nat));
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < VT; ++j) {
+        const uint32_t i = tid + j * NT;
+        v[j] = K{};
+        if (i < ntot) v[j] = *(i < nat ? at + i : bt + (i - nat));
+      }
     }
     if (tid < next) ev = bx[tid];
 #pragma unroll
     for (int j = 0; j < VT; ++j) {
       const uint32_t i = tid + j * NT;
-      if (i < nat + nbt) keys[i] = v[j];
+      if (i < ntot) keys[i] = v[j];
     }
-    if (tid < next) keys[nat + nbt + tid] = ev;
+    if (tid < next) keys[ntot + tid] = ev;
   }
   __syncthreads();
   const K* sa = keys;
@@ -158,12 +191,12 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
   const uint32_t nbx = nbt + next;  // right keys in LDS
   int32_t* __restrict__ lbt = lower_bound + a0;
 
-  // Left key i = tid + j * NT; JA (block-uniform) bounds j.  The lower
-  // bound by a fixed-step binary search of the tile's right keys; the
+  // Left key i = tid + j * NT; JA (uniform over the group) bounds j.  The
+  // lower bound by a fixed-step binary search of the tile's right keys; the
   // equal run after it by a fixed-step search of the next kRunWin right
   // keys, widened to all of them only for a wave in which some run is
   // longer.
-  constexpr uint32_t kRunWin = 64;
+  constexpr uint32_t kRunWin = 16;
   const uint32_t JA = (nat + NT - 1) / NT;
   const uint32_t top_lo = nbt ? 1u << (31 - __builtin_clz(nbt)) : 0u;
   uint32_t cnt[VT];
@@ -247,44 +280,64 @@ __global__ __launch_bounds__(NT) void k_find_bounds(
   }
   __syncthreads();
 
-  // Exclusive scan of the counts in left-key order: thread t owns
-  // elements [t*VT, t*VT + VT).
+  // Exclusive scan of the counts in left-key order within the tile: thread
+  // t of the group owns elements [t*VT, t*VT + VT).  They are read again
+  // afterwards rather than held in registers through the look-back.
   const uint32_t c0 = tid * VT;
-  uint32_t cv[VT];
   uint64_t my_sum = 0;
 #pragma unroll
-  for (int j = 0; j < VT; ++j) {
-    cv[j] = (c0 + j < nat) ? s_cnt[cnt_slot(c0 + j)] : 0u;
-    my_sum += cv[j];
+  for (int j = 0; j < VT; ++j) my_sum += (c0 + j < nat) ? s_cnt[cnt_slot(c0 + j)] : 0u;
+  const uint32_t w = threadIdx.x / kWave;
+  const uint64_t incl = wave_incl_scan(my_sum);
+  if (lane_id() == kWave - 1) s_wave[w] = incl;
+  __syncthreads();
+  uint64_t run = incl - my_sum;  // within the tile: waves before this one
+  uint64_t sub_sum = 0;
+#pragma unroll
+  for (int i = 0; i < NWS; ++i) {
+    const uint64_t x = s_wave[sub * NWS + i];
+    run += (static_cast<uint32_t>(i) < w % NWS) ? x : 0ull;
+    sub_sum += x;
   }
-  uint64_t tile_sum;
-  uint64_t run = block_excl_scan<NT>(my_sum, scan_tmp, &tile_sum);
+  if (tid == 0) s_sub[sub] = sub_sum;
+  __syncthreads();
+  uint64_t before = 0, blk_sum = 0;  // tiles of this block before this one; all of them
+#pragma unroll
+  for (int i = 0; i < kFbSub; ++i) {
+    before += (static_cast<uint32_t>(i) < sub) ? s_sub[i] : 0ull;
+    blk_sum += s_sub[i];
+  }
 
-  // Tile prefix by decoupled look-back (wave 0).
-  if (tid < kWave) {
-    if (tile == 0) {
-      if (tid == 0) status_store64(status, kFlagIncl64 | tile_sum);
-      if (tid == 0) s_prefix = 0;
+  // Block prefix by decoupled look-back (wave 0).
+  if (threadIdx.x < kWave) {
+    if (blk == 0) {
+      if (threadIdx.x == 0) {
+        status_store64(status, kFlagIncl64 | blk_sum);
+        s_prefix = 0;
+      }
     } else {
-      if (tid == 0) status_store64(status + tile, kFlagAgg64 | tile_sum);
-      uint64_t excl = wave_lookback64(status, tile);
-      if (tid == 0) {
-        status_store64(status + tile, kFlagIncl64 | (excl + tile_sum));
+      if (threadIdx.x == 0) status_store64(status + blk, kFlagAgg64 | blk_sum);
+      const uint64_t excl = wave_lookback64(status, blk);
+      if (threadIdx.x == 0) {
+        status_store64(status + blk, kFlagIncl64 | (excl + blk_sum));
         s_prefix = excl;
       }
     }
   }
 #pragma unroll
   for (int j = 0; j < VT; ++j) {
-    if (c0 + j < nat) s_cnt[cnt_slot(c0 + j)] = static_cast<uint32_t>(run);  // tile-local prefix
-    run += cv[j];
+    if (c0 + j < nat) {
+      const uint32_t c = s_cnt[cnt_slot(c0 + j)];
+      s_cnt[cnt_slot(c0 + j)] = static_cast<uint32_t>(run);  // tile-local prefix
+      run += c;
+    }
   }
   __syncthreads();
-  const uint64_t prefix = s_prefix;
+  const uint64_t prefix = s_prefix + before;
   int32_t* __restrict__ lct = left_count + a0;
   for (uint32_t i = tid; i < nat; i += NT)
     lct[i] = static_cast<int32_t>(prefix + s_cnt[cnt_slot(i)]);
-  if (tile == ntiles - 1 && tid == 0) *total = prefix + tile_sum;
+  if (tile == ntiles - 1 && tid == 0) *total = prefix + sub_sum;
 }
 
 // ---------------------------------------------------------------------------
@@ -548,10 +601,11 @@ __global__ __launch_bounds__(256) void k_gather(GatherSpec gs, const int32_t* __
 }
 
 // Scratch of one find_bounds launch over `ntiles` tiles: the merge-path
-// splits, the look-back status words and the tile ticket, each aligned to
-// 256 bytes.
+// splits, the look-back status words (one per block) and the block ticket,
+// each aligned to 256 bytes.
+constexpr uint64_t fb_blocks(uint64_t ntiles) { return (ntiles + kFbSub - 1) / kFbSub; }
 constexpr uint64_t fb_scratch_bytes(uint64_t ntiles) {
-  return align_up(align_up(8 * (ntiles + 1), 256) + 8 * ntiles, 256) + 4;
+  return align_up(align_up(8 * (ntiles + 1), 256) + 8 * fb_blocks(ntiles), 256) + 4;
 }
 
 // `scratch` is the device memory the launch uses for fb_scratch_bytes():
@@ -578,7 +632,8 @@ int find_bounds_nv(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, cons
   }
   Carve w(scratch);
   uint64_t* split = w.take<uint64_t>(ntiles + 1);
-  uint64_t* status = w.take<uint64_t>(ntiles);
+  const uint64_t nblk = fb_blocks(ntiles);
+  uint64_t* status = w.take<uint64_t>(nblk);
   uint32_t* ticket = w.take<uint32_t>(1);
   // status and ticket are adjacent after alignment; zero both.
   RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(status),
@@ -588,7 +643,7 @@ int find_bounds_nv(rfx_ctx* ctx, int32_t* lb, int32_t* lc, uint64_t* total, cons
   RFX_LAUNCHED();
   const bool prof = own && ctx->profiling;
   if (prof) prof_begin(ctx, "join.find_bounds");
-  k_find_bounds<K, kFbNT, NV><<<static_cast<uint32_t>(ntiles), kFbNT, 0, ctx->stream>>>(
+  k_find_bounds<K, kFbNT, NV><<<static_cast<uint32_t>(nblk), kFbNT * kFbSub, 0, ctx->stream>>>(
       static_cast<const K*>(left), nl, static_cast<const K*>(right), nr, split,
       static_cast<uint32_t>(ntiles), lb, lc, total, status, ticket);
   if (prof) prof_end(ctx);

@@ -246,3 +246,4 @@ def test_hbm_scale_shape_against_oracle(rf, ctx, orc):
     left = sorted_keys(orc, 4, 1 << 20, 301, 1 << 20)
     right = sorted_keys(orc, 4, 1 << 22, 302, 1 << 20)
     assert check_join(rf, ctx, orc, left, right, 4) > (1 << 21)
+
