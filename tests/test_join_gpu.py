@@ -247,3 +247,59 @@ def test_hbm_scale_shape_against_oracle(rf, ctx, orc):
     right = sorted_keys(orc, 4, 1 << 22, 302, 1 << 20)
     assert check_join(rf, ctx, orc, left, right, 4) > (1 << 21)
 
+
+
+def test_config1_plan_replayed_from_a_graph(rf, ctx, orc):
+    """The config-1 plan (restore the unsorted inputs, sort both sides,
+    find_bounds, expansion with gather) captured once into a HIP graph:
+    every replay, with eager work and synchronisations between replays,
+    writes the tuples the eager plan wrote."""
+    n_r, n_s = 65536, 262144
+    rk, rv, sk, sv = datagen.join_tables(datagen.BASE_SEED + 40, n_r, n_s, 16)
+    src = [rf.DeviceBuffer.from_numpy(x) for x in (rk, rv, sk, sv)]
+    work = [rf.DeviceBuffer(x.nbytes) for x in src]
+    lb, lc, tot = rf.DeviceBuffer(4 * n_r), rf.DeviceBuffer(4 * n_r), rf.u64_cell()
+    stream = rf.Stream()
+    gctx = rf.Context(0, stream.handle)  # capture needs a stream of its own
+
+    def enqueue(total):
+        for w, s in zip(work, src):
+            gctx.copy(w, s, s.nbytes)
+        gctx.sort_pairs(work[0], work[1], n_r, 4, 32, 4)
+        gctx.sort_pairs(work[2], work[3], n_s, 4, 32, 4)
+        gctx.join_find_bounds(lb, lc, tot, work[0], n_r, work[2], n_s, 4)
+        if total:
+            gctx.join_expand_gather(out, 16, work[0], 4, work[1], 4, work[3], 4, lb, lc, n_r,
+                                    tot, total, 64, 32)
+
+    enqueue(0)
+    gctx.synchronize()
+    total = rf.read_u64(tot)
+    out = rf.DeviceBuffer(16 * total)
+    enqueue(total)  # also sizes the context's workspace before the capture
+    gctx.synchronize()
+    want = out.download(np.uint64, 2 * total)
+    # the eager tuples against the oracle's join of the same tables
+    ok, ov = orc.sort_pairs(rk, rv, 4, 32, 4)
+    sk_s, sv_s = orc.sort_pairs(sk, sv, 4, 32, 4)
+    olb, olc, ot = orc.join_find_bounds(ok, sk_s, 4)
+    oli, ori = orc.join_main(olb, olc, ot)
+    assert ot == total
+    tup = np.empty(2 * total, dtype=np.uint64)
+    tup[0::2] = (ov[oli].astype(np.uint64) << np.uint64(32)) | sv_s[ori].astype(np.uint64)
+    tup[1::2] = ok[oli].astype(np.uint64)
+    assert np.array_equal(want, tup)
+    gctx.graph_begin()
+    enqueue(total)
+    g = gctx.graph_end()
+    for _ in range(3):
+        out.zero()
+        for w in work:
+            w.zero()
+        gctx.synchronize()
+        g.launch()
+        gctx.synchronize()
+        assert np.array_equal(out.download(np.uint64, 2 * total), want)
+    g.close()
+    gctx.close()
+    stream.close()
