@@ -518,6 +518,19 @@ def bench_select(rf, ctx, iters=5):
     kept = rf.read_u64(nout)
     res = {"n": n, "kept": kept, "ms": median(times[1:]), "rows_per_s": n / (median(times[1:]) * 1e-3),
            "roofline": roofline(16 * n + 16 * kept, launches, ms)}
+    # The autotuner's choice of rows per thread for 16-, 8- and 4-byte
+    # tuples of the same bytes (a 32-bit field < 2^31: half kept).
+    from redfox_amd.autotune import tune
+
+    res["autotune"] = {}
+    for w in (16, 8, 4):
+        m = 16 * n // w
+        sh = 64 if w == 16 else 0
+        t = tune(ctx, lambda: ctx.select(out, nout, tup, m, w, sh, 32, 2, 1 << 31),
+                 knobs={"select.vt": (0, 8, 16, 24, 32)}, reps=3)
+        res["autotune"][f"{w}B"] = {"choice": t["choice"]["select.vt"],
+                                    "ms": t["timings_ms"]["select.vt"]}
+    ctx.tune("select.vt", 0)
     for b in (tup, out, nout):
         b.close()
     return res
@@ -668,6 +681,8 @@ def summary(name, res):
                 if k in ("ms", "core_ms", "plan_ms", "plan_graph_ms", "batched_ms", "speedup",
                          "speedup_qps"):
                     parts.append(f"{prefix}{k}={v:.4g}")
+                elif k == "choice":
+                    parts.append(f"{prefix}{k}={v}")
                 continue
             if k == "roofline" or k.endswith("_roofline") or k.endswith("_pass"):
                 tag = prefix + (k[: -len("_roofline")] if k.endswith("_roofline") else

@@ -1,7 +1,8 @@
 """Autotuner for operator tile shapes.
 
 The kernels have a few compiled tile shapes (keys per thread of the radix
-sort's onesweep tile, merged items per find_bounds and expansion tile);
+sort's onesweep tile and its digit width, merged items per find_bounds and
+expansion tile, rows per thread of the select pass);
 which is fastest depends on the key and value widths, the row counts and
 the key distribution.  tune() times a whole plan (a callable that enqueues
 it on the context's stream) under each candidate of one knob at a time,
@@ -23,8 +24,10 @@ from . import rfx as R
 # knob -> candidates, the built-in default (0) first
 KNOBS: Dict[str, Tuple[int, ...]] = {
     "sort.ipt": (0, 8, 12, 16),
+    "sort.digit": (0, 8, 9, 10),
     "join.fb_tile": (0, 2048),
     "join.ex_tile": (0, 2048),
+    "select.vt": (0, 8, 16, 24, 32),
 }
 
 
@@ -43,10 +46,13 @@ def time_plan(ctx: R.Context, run: Callable[[], None], reps: int = 3) -> float:
 
 
 def tune(ctx: R.Context, run: Callable[[], None], knobs: Optional[Dict[str, Iterable[int]]] = None,
-         reps: int = 3, cache_path: Optional[str] = None, key: Optional[str] = None) -> dict:
+         reps: int = 3, cache_path: Optional[str] = None, key: Optional[str] = None,
+         margin: float = 0.01) -> dict:
     """Pick each knob's fastest value for run(); returns {knob: value} plus
-    the timings.  With cache_path and key, a stored choice is applied
-    without timing, and a new one is stored."""
+    the timings.  A knob keeps its first candidate (the built-in default)
+    unless another is faster by more than `margin` (a fraction of its time),
+    so timing noise does not move it.  With cache_path and key, a stored
+    choice is applied without timing, and a new one is stored."""
     knobs = dict(KNOBS if knobs is None else knobs)
     if cache_path and key:
         cache = _load(cache_path)
@@ -59,10 +65,13 @@ def tune(ctx: R.Context, run: Callable[[], None], knobs: Optional[Dict[str, Iter
     choice: Dict[str, int] = {}
     for knob, values in knobs.items():
         res = {}
+        values = [int(v) for v in values]
         for v in values:
             ctx.tune(knob, v)
-            res[int(v)] = time_plan(ctx, run, reps)
+            res[v] = time_plan(ctx, run, reps)
         best = min(res, key=res.get)
+        if res[best] > res[values[0]] * (1.0 - margin):
+            best = values[0]
         ctx.tune(knob, best)
         choice[knob] = best
         timings[knob] = res

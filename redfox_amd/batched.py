@@ -223,9 +223,11 @@ class BatchedRunner:
     def autotune(self, reps: int = 2) -> dict:
         """Choose the operators' tile shapes for this workload by timing the
         whole plan (redfox_amd.autotune); capture() afterwards records them."""
-        from .autotune import tune
+        from .autotune import KNOBS, tune
 
-        return tune(self.ctx, self._enqueue, reps=reps)
+        # The plan has no select: its knob would only time noise.
+        knobs = {k: v for k, v in KNOBS.items() if not k.startswith("select.")}
+        return tune(self.ctx, self._enqueue, knobs=knobs, reps=reps)
 
     def capture(self) -> None:
         """Run once eagerly (sizes the workspace), then record the graph."""

@@ -305,6 +305,12 @@ static int* tune_field(rfx_ctx* ctx, const char* knob, const int** allowed, int*
     *nallowed = 4;
     return &ctx->tune_sort_ipt;
   }
+  if (!std::strcmp(knob, "sort.digit")) {
+    static const int digits[] = {0, 8, 9, 10};
+    *allowed = digits;
+    *nallowed = 4;
+    return &ctx->tune_sort_digit;
+  }
   if (!std::strcmp(knob, "join.fb_tile")) {
     *allowed = tiles;
     *nallowed = 3;
@@ -314,6 +320,12 @@ static int* tune_field(rfx_ctx* ctx, const char* knob, const int** allowed, int*
     *allowed = tiles;
     *nallowed = 3;
     return &ctx->tune_ex_tile;
+  }
+  if (!std::strcmp(knob, "select.vt")) {
+    static const int vts[] = {0, 8, 16, 24, 32};
+    *allowed = vts;
+    *nallowed = 5;
+    return &ctx->tune_sel_vt;
   }
   if (!std::strcmp(knob, "mapfilter.mode")) {
     *allowed = mf_modes;

@@ -30,8 +30,10 @@ struct rfx_ctx {
   uint64_t* cells = nullptr;  // kCtxCells device u64 cells for scalar operands
   // Tile shapes chosen by rfx_ctx_tune (0: the built-in default).
   int tune_sort_ipt = 0;      // keys per thread of the onesweep tile
+  int tune_sort_digit = 0;    // radix digit bits (0: chosen from the key bits)
   int tune_fb_tile = 0;       // merged keys per find_bounds tile
   int tune_ex_tile = 0;       // merged items per expansion tile
+  int tune_sel_vt = 0;        // rows per thread of the select pass
   int tune_mf_mode = 0;       // MAPFILTER: 0 compiled (else interpreted), 1 compiled, 2 interpreted
   std::vector<rfx_prof_rec> prof_pending;
   std::vector<hipEvent_t> prof_pool;

@@ -26,10 +26,13 @@ namespace {
 #endif
 constexpr int kFNT = RFX_SEL_NT;
 constexpr int kFNW = kFNT / kWave;
-// Rows per thread: 16, and 24 for 16-byte tuples (2^27 tuples, half kept:
-// 0.64 ms against 0.69 ms with 16 and 0.75 ms with 32).
+// Rows per thread by default, from the autotuner's sweep over the same
+// 2 GiB of tuples with half kept (ms for 8 / 16 / 24 / 32 rows): 16-byte
+// tuples 0.74 / 0.68 / 0.63 / 0.74, 8-byte 1.06 / 0.84 / 0.72 / 0.78,
+// 4-byte 1.71 / 1.24 / 1.05 / 0.96.  The select.vt knob picks 8, 16, 24 or
+// 32 instead.
 template <class K>
-constexpr int kFVT = sizeof(K) >= 16 ? RFX_SEL_VT16 : 16;
+constexpr int kFVT = sizeof(K) >= 16 ? RFX_SEL_VT16 : sizeof(K) == 8 ? 24 : 32;
 
 template <class K>
 __device__ __forceinline__ u128 widen(K v) { return to128(v); }
@@ -73,13 +76,12 @@ struct Pred {
   uint64_t k;
 };
 
-template <class K>
+template <class K, int VT>
 __global__ __launch_bounds__(kFNT) void k_select(const K* __restrict__ in, uint64_t n,
                                                  uint32_t ntiles, Pred p, K* __restrict__ out,
                                                  uint64_t* __restrict__ n_out,
                                                  uint64_t* __restrict__ status,
                                                  uint32_t* __restrict__ ticket) {
-  constexpr int VT = kFVT<K>;
   constexpr int TILE = kFNT * VT;
   constexpr int kE = VT * kFNW, P = kE > kWave ? kE / kWave : 1;
   static_assert(kE <= kWave || kE % kWave == 0, "whole runs of step counts per lane");
@@ -223,10 +225,9 @@ uint32_t stream_grid(rfx_ctx* ctx, uint64_t n) {
   return static_cast<uint32_t>(b ? b : 1);
 }
 
-template <class K>
-int select_impl(rfx_ctx* ctx, void* out, uint64_t* n_out, const void* in, uint64_t n, Pred p) {
-  if (n == 0) return fill_u32(ctx, reinterpret_cast<uint32_t*>(n_out), 2, 0);
-  constexpr uint64_t TILE = static_cast<uint64_t>(kFNT) * kFVT<K>;
+template <class K, int VT>
+int select_vt(rfx_ctx* ctx, void* out, uint64_t* n_out, const void* in, uint64_t n, Pred p) {
+  constexpr uint64_t TILE = static_cast<uint64_t>(kFNT) * VT;
   const uint64_t ntiles = (n + TILE - 1) / TILE;
   Carve c(nullptr);
   c.take<uint64_t>(ntiles);
@@ -238,11 +239,24 @@ int select_impl(rfx_ctx* ctx, void* out, uint64_t* n_out, const void* in, uint64
   RFX_TRY(fill_u32(ctx, reinterpret_cast<uint32_t*>(status),
                    (reinterpret_cast<char*>(ticket + 1) - reinterpret_cast<char*>(status)) / 4, 0));
   RFX_PROF(ctx, "select");
-  k_select<K><<<static_cast<uint32_t>(ntiles), kFNT, 0, ctx->stream>>>(
+  k_select<K, VT><<<static_cast<uint32_t>(ntiles), kFNT, 0, ctx->stream>>>(
       static_cast<const K*>(in), n, static_cast<uint32_t>(ntiles), p, static_cast<K*>(out), n_out,
       status, ticket);
   RFX_LAUNCHED();
   return RFX_OK;
+}
+
+// Rows per thread: the built-in kFVT<K>, or the context's select.vt knob.
+template <class K>
+int select_impl(rfx_ctx* ctx, void* out, uint64_t* n_out, const void* in, uint64_t n, Pred p) {
+  if (n == 0) return fill_u32(ctx, reinterpret_cast<uint32_t*>(n_out), 2, 0);
+  switch (ctx->tune_sel_vt) {
+    case 8: return select_vt<K, 8>(ctx, out, n_out, in, n, p);
+    case 16: return select_vt<K, 16>(ctx, out, n_out, in, n, p);
+    case 24: return select_vt<K, 24>(ctx, out, n_out, in, n, p);
+    case 32: return select_vt<K, 32>(ctx, out, n_out, in, n, p);
+  }
+  return select_vt<K, kFVT<K>>(ctx, out, n_out, in, n, p);
 }
 
 }  // namespace

@@ -1,8 +1,11 @@
-// sort.hip - stable LSD radix sort for gfx950: 8-bit digits, one upfront
-// histogram pass over the keys for all digit positions, then one
-// "onesweep" kernel per digit that ranks a tile in registers/LDS, obtains
-// its global digit offsets by decoupled look-back and scatters through an
-// LDS reorder buffer.
+// sort.hip - stable LSD radix sort for gfx950: 8-bit digits (9 or 10 bits
+// when a context is tuned to them), one upfront histogram pass over the
+// keys for all digit positions, then one "onesweep" kernel per digit that
+// ranks a tile in registers/LDS, obtains its global digit offsets by
+// decoupled look-back and scatters through an LDS reorder buffer.  Wider
+// digits take fewer passes over the data (26-bit keys: 3 instead of 4;
+// 20-bit: 2 instead of 3) at the cost of more bins per tile: more ranking
+// ballots and look-back words, shorter runs per bin in the scatter.
 //
 // Replaces redfox::sort_pair / sort_key (b40cSort.cu:58-1021: b40c LSD
 // radix, 4-5 bit digits, upsweep + spine + downsweep per digit, pong
@@ -23,7 +26,10 @@
 namespace rfx {
 namespace {
 
-constexpr int kBins = 256;
+// Digit width DB: 1 << DB bins.  8 by default; 9 and 10 for the
+// sort.digit knob.
+template <int DB>
+constexpr int kBinsOf = 1 << DB;
 
 #ifndef RFX_SORT_NT
 #define RFX_SORT_NT 512
@@ -52,23 +58,23 @@ struct SortTune {
 };
 
 struct SortLayout {
-  uint32_t* hist;    // [P][256]
-  uint32_t* base;    // [P][256] exclusive global digit offsets
+  uint32_t* hist;    // [P][bins]
+  uint32_t* base;    // [P][bins] exclusive global digit offsets
   uint32_t* ctrs;    // [P][2]   tile ticket, finished tiles
   uint32_t* skip;    // [P]
   uint32_t* sel;     // [1]      0: data in caller's buffers, 1: in pong
-  uint32_t* status;  // [P][tiles][256]
+  uint32_t* status;  // [P][tiles][bins]
   void* pong_keys;
   void* pong_vals;
   uint64_t zero_words;  // words from hist to the end of status
 };
 
 static void carve_sort(Carve& c, uint64_t n, int kb, int vb, int passes, uint64_t tiles,
-                       SortLayout* L) {
+                       uint32_t bins, SortLayout* L) {
   // The zeroed block comes first so it is one contiguous range.
-  const uint64_t status_words = static_cast<uint64_t>(passes) * tiles * kBins;
-  L->hist = c.take<uint32_t>(static_cast<uint64_t>(passes) * kBins);
-  L->base = c.take<uint32_t>(static_cast<uint64_t>(passes) * kBins);
+  const uint64_t status_words = static_cast<uint64_t>(passes) * tiles * bins;
+  L->hist = c.take<uint32_t>(static_cast<uint64_t>(passes) * bins);
+  L->base = c.take<uint32_t>(static_cast<uint64_t>(passes) * bins);
   L->ctrs = c.take<uint32_t>(static_cast<uint64_t>(passes) * 2);
   L->skip = c.take<uint32_t>(static_cast<uint64_t>(passes));
   L->sel = c.take<uint32_t>(1);
@@ -78,28 +84,43 @@ static void carve_sort(Carve& c, uint64_t n, int kb, int vb, int passes, uint64_
   L->pong_vals = vb ? c.take<char>(n * vb) : nullptr;
 }
 
+template <int DB>
 __device__ __forceinline__ uint32_t pass_mask(int pass, uint32_t key_bits) {
-  uint32_t width = key_bits - 8u * static_cast<uint32_t>(pass);
-  if (width > 8) width = 8;
+  uint32_t width = key_bits - static_cast<uint32_t>(DB * pass);
+  if (width > DB) width = DB;
   return (1u << width) - 1u;
+}
+
+// Digit of key k at `shift`.  8-bit digits never straddle a 32-bit half of
+// a 64-bit key (digit_of's fast form); wider ones may, and take the
+// 64-bit shift.
+template <int DB, class K>
+__device__ __forceinline__ uint32_t dig(K k, uint32_t shift, uint32_t m) {
+  if constexpr (DB == 8 || sizeof(K) <= 4) {
+    return digit_of(k, shift, m);
+  } else {
+    static_assert(sizeof(K) == 8, "digits wider than 8 bits are for keys of up to 8 bytes");
+    return static_cast<uint32_t>(k >> shift) & m;
+  }
 }
 
 // Histogram of every digit position in one read of the keys.  With an
 // aligned key pointer each thread reads 16 bytes at a time.
-template <int KB, bool VEC>
+template <int KB, bool VEC, int DB>
 __global__ __launch_bounds__(256) void k_radix_hist(const typename Elem<KB>::T* __restrict__ keys,
                                                     uint64_t n, int passes, uint32_t key_bits,
                                                     uint32_t* __restrict__ hist) {
   using K = typename Elem<KB>::T;
+  constexpr int kBins = kBinsOf<DB>;
   constexpr int PER = (VEC && KB < 16) ? 16 / KB : 1;
-  __shared__ uint32_t lh[16 * kBins];
+  __shared__ uint32_t lh[(8 * KB + DB - 1) / DB * kBins];
   for (int i = threadIdx.x; i < passes * kBins; i += 256) lh[i] = 0;
   __syncthreads();
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
   const uint64_t gtid = blockIdx.x * 256ull + threadIdx.x;
   auto count = [&](K k) {
     for (int p = 0; p < passes; ++p)
-      atomicAdd(&lh[p * kBins + digit_of(k, 8u * p, pass_mask(p, key_bits))], 1u);
+      atomicAdd(&lh[p * kBins + dig<DB>(k, DB * p, pass_mask<DB>(p, key_bits))], 1u);
   };
   uint64_t done = 0;
   if constexpr (PER > 1) {
@@ -113,11 +134,11 @@ __global__ __launch_bounds__(256) void k_radix_hist(const typename Elem<KB>::T* 
       const Vec x = kv[v];
       const uint64_t active = ballot(true);
       for (int p = 0; p < passes; ++p) {
-        const uint32_t sh = 8u * p, m = pass_mask(p, key_bits);
-        const uint32_t d = digit_of(x.k[0], sh, m);
+        const uint32_t sh = DB * p, m = pass_mask<DB>(p, key_bits);
+        const uint32_t d = dig<DB>(x.k[0], sh, m);
         bool same = true;
 #pragma unroll
-        for (int e = 1; e < PER; ++e) same &= digit_of(x.k[e], sh, m) == d;
+        for (int e = 1; e < PER; ++e) same &= dig<DB>(x.k[e], sh, m) == d;
         if (ballot(same) == active) {
           const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
           if (ballot(d != d0) == 0) {
@@ -130,7 +151,7 @@ __global__ __launch_bounds__(256) void k_radix_hist(const typename Elem<KB>::T* 
           atomicAdd(&lh[p * kBins + d], static_cast<uint32_t>(PER));
         } else {
 #pragma unroll
-          for (int e = 0; e < PER; ++e) atomicAdd(&lh[p * kBins + digit_of(x.k[e], sh, m)], 1u);
+          for (int e = 0; e < PER; ++e) atomicAdd(&lh[p * kBins + dig<DB>(x.k[e], sh, m)], 1u);
         }
       }
     }
@@ -142,20 +163,24 @@ __global__ __launch_bounds__(256) void k_radix_hist(const typename Elem<KB>::T* 
     if (lh[i]) atomicAdd(&hist[i], lh[i]);
 }
 
-// Exclusive scan of each digit histogram; flags identity passes.
-__global__ __launch_bounds__(256) void k_radix_scan(const uint32_t* __restrict__ hist,
-                                                    uint32_t* __restrict__ base,
-                                                    uint32_t* __restrict__ skip, uint64_t n) {
-  __shared__ uint32_t tmp[8];
+// Exclusive scan of each digit histogram (one thread per bin); flags
+// identity passes.
+template <int DB>
+__global__ __launch_bounds__(kBinsOf<DB>) void k_radix_scan(const uint32_t* __restrict__ hist,
+                                                            uint32_t* __restrict__ base,
+                                                            uint32_t* __restrict__ skip,
+                                                            uint64_t n) {
+  constexpr int kBins = kBinsOf<DB>;
+  __shared__ uint32_t tmp[kBins / kWave + 1];
   const int p = blockIdx.x;
   const uint32_t c = hist[p * kBins + threadIdx.x];
   uint32_t total;
-  uint32_t e = block_excl_scan<256>(c, tmp, &total);
+  uint32_t e = block_excl_scan<kBins>(c, tmp, &total);
   base[p * kBins + threadIdx.x] = e;
   if (c == n) skip[p] = 1u;  // every key has the same digit here
 }
 
-template <int KB, int VB, int NT, int IPT>
+template <int KB, int VB, int NT, int IPT, int DB>
 __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
     typename Elem<KB>::T* __restrict__ k0, typename Elem<KB>::T* __restrict__ k1,
     typename Elem<VB>::T* __restrict__ v0, typename Elem<VB>::T* __restrict__ v1, uint64_t n,
@@ -167,6 +192,13 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
   constexpr int NW = NT / kWave;
   constexpr int TILE = NT * IPT;
   constexpr int RB = (KB > VB ? KB : VB) * TILE;
+  constexpr int kBins = kBinsOf<DB>;
+  constexpr int BPT = (kBins + NT - 1) / NT;  // bins per thread in the per-bin steps
+  // Wave counters and tile positions stay below TILE: 16 bits hold them,
+  // which keeps the wider digits' counters within LDS.
+  using WC = typename std::conditional<(DB > 8), uint16_t, uint32_t>::type;
+  using DG = typename std::conditional<(DB > 8), uint16_t, uint8_t>::type;
+  static_assert(TILE <= 65535 || DB == 8, "16-bit wave counters");
 
   if (skip[pass]) return;
   const uint32_t cur = *sel;
@@ -176,17 +208,17 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
   V* __restrict__ vout = cur ? v0 : v1;
 
   __shared__ uint32_t s_tile;
-  __shared__ uint32_t wcnt[NW][kBins];
+  __shared__ WC wcnt[NW][kBins];
   __shared__ uint32_t gdelta[kBins];
   __shared__ uint32_t scan_tmp[NW + 1];
   __shared__ __attribute__((aligned(16))) unsigned char rbuf[RB];
-  __shared__ uint8_t sdig[VB > 0 ? TILE : 1];  // digit of each reordered row
+  __shared__ DG sdig[VB > 0 ? TILE : 1];  // digit of each reordered row
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t w = tid / kWave;
-  const uint32_t shift = 8u * static_cast<uint32_t>(pass);
-  const uint32_t dmask = pass_mask(pass, key_bits);
+  const uint32_t shift = static_cast<uint32_t>(DB * pass);
+  const uint32_t dmask = pass_mask<DB>(pass, key_bits);
 
   if (tid == 0) s_tile = atomicAdd(&ctrs[pass * 2], 1u);
   for (uint32_t i = tid; i < NW * kBins; i += NT) (&wcnt[0][0])[i] = 0;
@@ -220,7 +252,7 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
     for (int j = 0; j < IPT; ++j) {
       const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
       const bool valid = FULL || idx < n;
-      const uint32_t d = digit_of(keys[j], shift, dmask);
+      const uint32_t d = dig<DB>(keys[j], shift, dmask);
       // acc collects, per lane, the lanes whose digit differs in some bit:
       // x is the lane's bit b as 0 or all ones, and acc | (bb ^ x) is one
       // bitop3 per 32-bit half (table 0xDE with acc in the middle operand
@@ -228,7 +260,7 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
       // above a narrow last digit are zero everywhere and change nothing.
       uint32_t acc_lo = 0, acc_hi = 0;
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
+      for (int b = 0; b < DB; ++b) {
         const uint32_t x = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(d), b, 1));
         // The ballot of x itself (a plain ballot(x != 0) is rewritten into
         // a second shift-and-compare of d); 33 is the not-equal predicate.
@@ -244,7 +276,7 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
       uint32_t prior = 0;
       if (valid) {
         prior = wcnt[w][d];
-        if (below == cnt - 1) wcnt[w][d] = prior + cnt;
+        if (below == cnt - 1) wcnt[w][d] = static_cast<WC>(prior + cnt);
       }
       pos[j] = prior + below;
     }
@@ -253,24 +285,42 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
     // Per-digit tile totals and wave prefixes; publish the tile aggregate.
     // wcnt[w][d] becomes the tile position of wave w's first key of digit
     // d: the digit's start in the tile plus the waves before w.
-    uint32_t total_d = 0;
-    uint32_t wpre[NW];
-    if (tid < kBins) {
-      uint32_t run = 0;
+    // Thread t handles bins t, t + NT, ... (one bin, or none, for 8 and 9
+    // bits).
+    uint32_t total_d[BPT], bs[BPT];
+    uint32_t wpre[BPT][NW];
 #pragma unroll
-      for (int ww = 0; ww < NW; ++ww) {
-        wpre[ww] = run;
-        run += wcnt[ww][tid];
+    for (int q = 0; q < BPT; ++q) {
+      const uint32_t bin = tid + q * NT;
+      total_d[q] = 0;
+      if (bin < kBins) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) {
+          wpre[q][ww] = run;
+          run += wcnt[ww][bin];
+        }
+        total_d[q] = run;
+        status_store(st + static_cast<uint64_t>(tile) * kBins + bin,
+                     (tile == 0 ? kFlagIncl : kFlagAgg) | run);
       }
-      total_d = run;
-      status_store(st + static_cast<uint64_t>(tile) * kBins + tid,
-                   (tile == 0 ? kFlagIncl : kFlagAgg) | total_d);
     }
-    uint32_t all;
-    const uint32_t bs = block_excl_scan<NT>(tid < kBins ? total_d : 0u, scan_tmp, &all);
-    if (tid < kBins) {
+    {
+      uint32_t carry = 0;
 #pragma unroll
-      for (int ww = 0; ww < NW; ++ww) wcnt[ww][tid] = bs + wpre[ww];
+      for (int q = 0; q < BPT; ++q) {
+        uint32_t all;
+        bs[q] = carry + block_excl_scan<NT>(total_d[q], scan_tmp, &all);
+        carry += all;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+      const uint32_t bin = tid + q * NT;
+      if (bin < kBins) {
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) wcnt[ww][bin] = static_cast<WC>(bs[q] + wpre[q][ww]);
+      }
     }
     __syncthreads();
 
@@ -282,7 +332,7 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
     for (int j = 0; j < IPT; ++j) {
       const uint64_t idx = wave_base + static_cast<uint64_t>(j) * kWave + lane;
       if (FULL || idx < n) {
-        const uint32_t d = digit_of(keys[j], shift, dmask);
+        const uint32_t d = dig<DB>(keys[j], shift, dmask);
         pos[j] += wcnt[w][d];
         kbuf[pos[j]] = keys[j];
       }
@@ -295,13 +345,18 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
         if (FULL || idx < n) vals[j] = vin[idx];
       }
     }
-    if (tid < kBins) {
-      uint32_t excl = 0;
-      if (tile > 0) {
-        excl = lookback_window<RFX_SORT_LBW>(st, tile, kBins, tid);
-        status_store(st + static_cast<uint64_t>(tile) * kBins + tid, kFlagIncl | (excl + total_d));
+#pragma unroll
+    for (int q = 0; q < BPT; ++q) {
+      const uint32_t bin = tid + q * NT;
+      if (bin < kBins) {
+        uint32_t excl = 0;
+        if (tile > 0) {
+          excl = lookback_window<RFX_SORT_LBW>(st, tile, kBins, bin);
+          status_store(st + static_cast<uint64_t>(tile) * kBins + bin,
+                       kFlagIncl | (excl + total_d[q]));
+        }
+        gdelta[bin] = base[pass * kBins + bin] + excl - bs[q];
       }
-      gdelta[tid] = base[pass * kBins + tid] + excl - bs;
     }
     __syncthreads();
 
@@ -313,8 +368,8 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
       const uint32_t i = k * NT + tid;
       if (FULL || i < tile_n) {
         const K key = kbuf[i];
-        const uint32_t d = digit_of(key, shift, dmask);
-        if constexpr (VB > 0) sdig[i] = static_cast<uint8_t>(d);
+        const uint32_t d = dig<DB>(key, shift, dmask);
+        if constexpr (VB > 0) sdig[i] = static_cast<DG>(d);
         const uint32_t dst = gdelta[d] + i;
         if (dst < n) kout[dst] = key;  // always true when ranks are right
       }
@@ -371,24 +426,45 @@ template <int KB, int VB>
 constexpr bool kSortTunable = (KB == 4 || KB == 8) && (VB == 0 || VB == 4 || VB == 8);
 constexpr int kSortMinIPT = 8;
 
-uint64_t sort_ws_tile(uint64_t n, int kb, int vb, int key_bits, uint64_t tile) {
-  const int passes = (key_bits + 7) / 8;
+uint64_t sort_ws_tile(uint64_t n, int kb, int vb, int key_bits, uint64_t tile, int db) {
+  const int passes = (key_bits + db - 1) / db;
   const uint64_t tiles = (n + tile - 1) / tile;
   Carve s(nullptr);
   SortLayout L;
-  carve_sort(s, n, kb, vb, passes, tiles, &L);
+  carve_sort(s, n, kb, vb, passes, tiles, 1u << db, &L);
   return s.off;
 }
 
-// Workspace for any tile shape the context may be tuned to.
+// Digit width chosen for key_bits when the context does not set one.  An
+// odd number of passes leaves the data in the pong buffer and costs a copy
+// back (about 0.87 ms for 2^28 u32 pairs, a full read and write), so 17-20
+// bit keys take two passes of 9 or 10 bits instead of three of 8: 2^28
+// pairs, 18 bits 3.36 ms against 5.04, 20 bits 4.04 against 4.90.  Wider
+// keys keep 8 bits (26 bits: 5.51 ms against 5.65 with 9; 32 bits: 5.56
+// against 6.29).
+constexpr int auto_digit(int key_bits) {
+  return key_bits >= 17 && key_bits <= 18 ? 9 : key_bits >= 19 && key_bits <= 20 ? 10 : 8;
+}
+
+// Workspace for any tile shape the context may be tuned to, at the digit
+// width chosen for key_bits (a context tuned to other digit widths grows
+// its own workspace on first use).
 template <int KB, int VB>
 uint64_t sort_ws_bytes(uint64_t n, int key_bits) {
   using Tn = SortTune<KB, VB>;
   const int ipt = kSortTunable<KB, VB> && kSortMinIPT < Tn::IPT ? kSortMinIPT : Tn::IPT;
-  return sort_ws_tile(n, KB, VB, key_bits, static_cast<uint64_t>(Tn::NT) * ipt);
+  uint64_t ws = sort_ws_tile(n, KB, VB, key_bits, static_cast<uint64_t>(Tn::NT) * ipt, 8);
+  if (kSortTunable<KB, VB> && auto_digit(key_bits) != 8) {
+    const int wide_ipt = Tn::IPT < 12 ? Tn::IPT : 12;
+    const uint64_t w2 = sort_ws_tile(n, KB, VB, key_bits,
+                                     static_cast<uint64_t>(Tn::NT) * wide_ipt,
+                                     auto_digit(key_bits));
+    if (w2 > ws) ws = w2;
+  }
+  return ws;
 }
 
-template <int KB, int VB, int IPT>
+template <int KB, int VB, int IPT, int DB = 8>
 int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
   using K = typename Elem<KB>::T;
   using V = typename Elem<VB>::T;
@@ -397,12 +473,13 @@ int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
   if (n < 2) return RFX_OK;
   if (n > kCountMask) return fail(RFX_ERANGE, "sort: %llu rows exceeds 2^30-1",
                                   static_cast<unsigned long long>(n));
-  const int passes = (key_bits + 7) / 8;
+  constexpr uint32_t kBins = kBinsOf<DB>;
+  const int passes = (key_bits + DB - 1) / DB;
   const uint64_t tiles = (n + TILE - 1) / TILE;
-  RFX_TRY(ensure_workspace(ctx, sort_ws_tile(n, KB, VB, key_bits, TILE)));
+  RFX_TRY(ensure_workspace(ctx, sort_ws_tile(n, KB, VB, key_bits, TILE, DB)));
   Carve c(ctx->ws);
   SortLayout L;
-  carve_sort(c, n, KB, VB, passes, tiles, &L);
+  carve_sort(c, n, KB, VB, passes, tiles, kBins, &L);
   RFX_TRY(fill_u32(ctx, L.hist, L.zero_words, 0u));
 
   uint64_t hb = (n + 256 * 16 - 1) / (256 * 16);
@@ -412,18 +489,18 @@ int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
   {
     RFX_PROF(ctx, "sort.hist");
     if (reinterpret_cast<uintptr_t>(keys) % 16 == 0)
-      k_radix_hist<KB, true><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
+      k_radix_hist<KB, true, DB><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
           static_cast<const K*>(keys), n, passes, static_cast<uint32_t>(key_bits), L.hist);
     else
-      k_radix_hist<KB, false><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
+      k_radix_hist<KB, false, DB><<<static_cast<uint32_t>(hb), 256, 0, ctx->stream>>>(
           static_cast<const K*>(keys), n, passes, static_cast<uint32_t>(key_bits), L.hist);
     RFX_LAUNCHED();
   }
-  k_radix_scan<<<passes, 256, 0, ctx->stream>>>(L.hist, L.base, L.skip, n);
+  k_radix_scan<DB><<<passes, kBins, 0, ctx->stream>>>(L.hist, L.base, L.skip, n);
   RFX_LAUNCHED();
   for (int p = 0; p < passes; ++p) {
     RFX_PROF(ctx, "sort.onesweep");
-    k_onesweep<KB, VB, NT, IPT><<<static_cast<uint32_t>(tiles), NT, 0, ctx->stream>>>(
+    k_onesweep<KB, VB, NT, IPT, DB><<<static_cast<uint32_t>(tiles), NT, 0, ctx->stream>>>(
         static_cast<K*>(keys), static_cast<K*>(L.pong_keys), static_cast<V*>(vals),
         static_cast<V*>(L.pong_vals), n, p, static_cast<uint32_t>(key_bits), L.base, L.ctrs,
         L.skip, L.sel, L.status, static_cast<uint32_t>(tiles));
@@ -441,6 +518,16 @@ int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
 template <int KB, int VB>
 int sort_tuned(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
   if constexpr (kSortTunable<KB, VB>) {
+    // Wider digits: the default tile, or 12 keys per thread.  Knob value 0
+    // takes the digit width chosen for key_bits.
+    constexpr int D = SortTune<KB, VB>::IPT;
+    const int digit = ctx->tune_sort_digit ? ctx->tune_sort_digit : auto_digit(key_bits);
+    if (digit == 9)
+      return ctx->tune_sort_ipt == 12 ? sort_impl<KB, VB, 12, 9>(ctx, keys, vals, n, key_bits)
+                                      : sort_impl<KB, VB, D, 9>(ctx, keys, vals, n, key_bits);
+    if (digit == 10)
+      return ctx->tune_sort_ipt == 12 ? sort_impl<KB, VB, 12, 10>(ctx, keys, vals, n, key_bits)
+                                      : sort_impl<KB, VB, D, 10>(ctx, keys, vals, n, key_bits);
     if (ctx->tune_sort_ipt == 8) return sort_impl<KB, VB, 8>(ctx, keys, vals, n, key_bits);
     if (ctx->tune_sort_ipt == 12) return sort_impl<KB, VB, 12>(ctx, keys, vals, n, key_bits);
     if (ctx->tune_sort_ipt == 16) return sort_impl<KB, VB, 16>(ctx, keys, vals, n, key_bits);

@@ -79,6 +79,23 @@ def test_select_keeps_all_or_none(rf, ctx, orc, tb, n):
         assert np.array_equal(got, want)
 
 
+
+@pytest.mark.parametrize("vt", [8, 16, 24, 32])
+@pytest.mark.parametrize("tb", [1, 4, 8, 16])
+def test_select_rows_per_thread_knob(rf, ctx, orc, tb, vt):
+    """Every select.vt tile shape keeps the oracle's rows in order: a
+    ragged last tile, many tiles, and a field in the tuple's top bits."""
+    n = 300_007
+    rows = rows_of(tb, n, 40 + tb)
+    bits = min(8 * tb, 16)
+    shift = 8 * tb - bits
+    try:
+        ctx.tune("select.vt", vt)
+        got = gpu_select(rf, ctx, rows, tb, shift, bits, 2, 1 << (bits - 1))
+    finally:
+        ctx.tune("select.vt", 0)
+    assert np.array_equal(got, orc.select(rows, tb, shift, bits, 2, 1 << (bits - 1)))
+
 def test_project_split_merge(rf, ctx, orc):
     n = 100_000
     rows = rows_of(8, n, 23)

@@ -153,3 +153,30 @@ def test_full_size_u64_zipf_pairs_properties(rf, ctx):
     assert np.all(v[1:][same] > v[:-1][same])
     assert np.array_equal(src[v], k)
     assert np.array_equal(np.sort(v), np.arange(n, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("digit", [8, 9, 10])
+@pytest.mark.parametrize("ipt", [0, 12])
+@pytest.mark.parametrize("kb,vb,bits", [(4, 0, 32), (4, 4, 18), (4, 4, 20), (4, 4, 26),
+                                        (4, 8, 31), (8, 4, 64), (8, 0, 37), (8, 8, 45)])
+def test_wider_digits_match_oracle(rf, ctx, orc, digit, ipt, kb, vb, bits):
+    """sort.digit 8, 9 and 10 forced over the width chosen from the key bits
+    (9 for 17-18 bits, 10 for 19-20): digits that straddle the 32-bit halves
+    of 64-bit keys, a narrow last digit, a skipped pass, a ragged last tile;
+    keys and values bit-exact against the oracle."""
+    n = 300_007
+    keys = make(kb, n, 700 + bits)
+    if bits == 26:
+        keys &= DT[kb](0x3FFFC00)  # bits 0-9 zero: the first pass is skipped
+    vals = make(vb, n, 800 + vb) if vb else None
+    try:
+        ctx.tune("sort.digit", digit)
+        ctx.tune("sort.ipt", ipt)
+        gk, gv = run_sort(rf, ctx, keys, vals, kb, bits, vb)
+    finally:
+        ctx.tune("sort.digit", 0)
+        ctx.tune("sort.ipt", 0)
+    ok, ov = orc.sort_pairs(keys, vals, kb, bits, vb)
+    assert np.array_equal(gk, ok)
+    if vb:
+        assert np.array_equal(gv, ov)
