@@ -82,6 +82,14 @@ __global__ __launch_bounds__(256) void k_fb_partition(const K* __restrict__ a, u
   split[t] = merge_path_lower(a, na, b, nb, diag);
 }
 
+// The largest key of each width: the sentinel that follows a tile's right
+// keys in LDS, so that fixed-step probes past them stay in the array.
+template <class K>
+__device__ __forceinline__ K max_key() {
+  if constexpr (sizeof(K) == 16) return K{~0ull, ~0ull};
+  else return static_cast<K>(~static_cast<K>(0));
+}
+
 // LDS index of count i: one pad word per 32, so that a thread reading its
 // VT consecutive counts and lanes writing thread-strided counts both stay
 // (nearly) free of bank conflicts.
@@ -120,7 +128,8 @@ __global__ __launch_bounds__(NT * kFbSub, 8) void k_find_bounds(
   static_assert(VT % 4 == 0, "left keys are searched four per thread at a time");
   constexpr int EXT = NT;
   constexpr int NWS = NT / kWave;  // waves per tile group
-  constexpr int KEY_BYTES = (NV + EXT) * static_cast<int>(sizeof(K));
+  constexpr int kRunWin = 31;      // run window: fixed steps 16, 8, 4, 2, 1
+  constexpr int KEY_BYTES = (NV + EXT + kRunWin + 1) * static_cast<int>(sizeof(K));
   constexpr int CNT_BYTES = (NV + NV / 32) * 4;
   constexpr int SUB_BYTES = ((KEY_BYTES > CNT_BYTES ? KEY_BYTES : CNT_BYTES) + 15) / 16 * 16;
   // The keys are dead once the searches are done; the counts reuse them.
@@ -133,7 +142,10 @@ __global__ __launch_bounds__(NT * kFbSub, 8) void k_find_bounds(
   if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t blk = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
-  const uint32_t sub = threadIdx.x / NT;                          // wave-uniform
+  // The tile group is the same for a whole wave (NT is a multiple of 64):
+  // readfirstlane lets the compiler keep the tile's bounds, and everything
+  // derived from them, in scalar registers.
+  const uint32_t sub = __builtin_amdgcn_readfirstlane(threadIdx.x / NT);
   const uint32_t tid = threadIdx.x % NT;
   const uint32_t tile = blk * kFbSub + sub;
   K* keys = reinterpret_cast<K*>(smem + sub * SUB_BYTES);
@@ -155,26 +167,26 @@ __global__ __launch_bounds__(NT * kFbSub, 8) void k_find_bounds(
 
   // The tile's A keys, its B keys, then the B keys past its end: every load
   // in flight before the first LDS write.  Merged index i reads A[a0 + i]
-  // below nat and B[b0 + i - nat] from there: one address select per key,
-  // and no bounds test at all in a full tile.
+  // below nat and B[b0 + i - nat] from there.  Row j of the tile (indices
+  // j*NT .. j*NT + NT - 1) lies wholly in A or wholly in B but for one row,
+  // so the side is a uniform branch and the address a scalar base plus the
+  // thread's offset.
   const K* __restrict__ at = a + a0;
   const K* __restrict__ bt = b + b0;
   const K* __restrict__ bx = b + b1;
   {
     K v[VT];
     K ev{};
-    if (ntot == NV) {
 #pragma unroll
-      for (int j = 0; j < VT; ++j) {
-        const uint32_t i = tid + j * NT;
-        v[j] = *(i < nat ? at + i : bt + (i - nat));
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < VT; ++j) {
-        const uint32_t i = tid + j * NT;
-        v[j] = K{};
-        if (i < ntot) v[j] = *(i < nat ? at + i : bt + (i - nat));
+    for (int j = 0; j < VT; ++j) {
+      const uint32_t i0 = j * NT, i = i0 + tid;
+      v[j] = K{};
+      if (i0 + NT <= nat) {
+        v[j] = at[i];
+      } else if (i0 >= nat && i0 + NT <= ntot) {
+        v[j] = bt[i - nat];
+      } else if (i < ntot) {
+        v[j] = i < nat ? at[i] : bt[i - nat];
       }
     }
     if (tid < next) ev = bx[tid];
@@ -184,6 +196,9 @@ __global__ __launch_bounds__(NT * kFbSub, 8) void k_find_bounds(
       if (i < ntot) keys[i] = v[j];
     }
     if (tid < next) keys[ntot + tid] = ev;
+    // Sentinels after the right keys: the run window's probes never leave
+    // the array, and a count is clamped to the keys actually there.
+    if (tid <= static_cast<uint32_t>(kRunWin)) keys[ntot + next + tid] = max_key<K>();
   }
   __syncthreads();
   const K* sa = keys;
@@ -192,13 +207,17 @@ __global__ __launch_bounds__(NT * kFbSub, 8) void k_find_bounds(
   int32_t* __restrict__ lbt = lower_bound + a0;
 
   // Left key i = tid + j * NT; JA (uniform over the group) bounds j.  The
-  // lower bound by a fixed-step binary search of the tile's right keys; the
-  // equal run after it by a fixed-step search of the next kRunWin right
-  // keys, widened to all of them only for a wave in which some run is
-  // longer.
-  constexpr uint32_t kRunWin = 16;
+  // lower bound by Shar's fixed-step binary search of the tile's right keys
+  // (one probe at the largest power of two P <= nbt picks the window [0, P)
+  // or [nbt - P, nbt), then steps P/2 .. 1 and a last probe); the equal run
+  // after it by a fixed-step search of the next kRunWin right keys, widened
+  // to all of them only for a wave in which some run is longer.  Positions
+  // are byte offsets into the right keys and the steps compile-time, so a
+  // probe is one LDS read at a constant offset, a compare and a select.
   const uint32_t JA = (nat + NT - 1) / NT;
   const uint32_t top_lo = nbt ? 1u << (31 - __builtin_clz(nbt)) : 0u;
+  constexpr uint32_t SZ = sizeof(K);
+  constexpr int kLogNV = __builtin_ctz(NV);
   uint32_t cnt[VT];
 #pragma unroll
   for (int j = 0; j < VT; ++j) cnt[j] = 0;
@@ -213,21 +232,45 @@ __global__ __launch_bounds__(NT * kFbSub, 8) void k_find_bounds(
         kv[q] = sa[i < nat ? i : nat - 1];
         lo[q] = 0;
       }
-      for (uint32_t step = top_lo; step; step >>= 1) {
+      const char* sbb = reinterpret_cast<const char*>(sb);
+      uint32_t off[4] = {0, 0, 0, 0};
+      if (top_lo) {
+        const K pivot = sb[top_lo - 1];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (lo[q] + step <= nbt && lt(sb[lo[q] + step - 1], kv[q])) lo[q] += step;
+        for (int q = 0; q < 4; ++q) off[q] = lt(pivot, kv[q]) ? (nbt - top_lo) * SZ : 0u;
+#pragma unroll
+        for (int sh = kLogNV - 1; sh >= 0; --sh) {
+          const uint32_t step = 1u << sh;
+          if (step < top_lo) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const K x = *reinterpret_cast<const K*>(sbb + off[q] + (step - 1) * SZ);
+              off[q] = lt(x, kv[q]) ? off[q] + step * SZ : off[q];
+            }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const K x = *reinterpret_cast<const K*>(sbb + off[q]);
+          off[q] = lt(x, kv[q]) ? off[q] + SZ : off[q];
+        }
       }
       // sb[lo, up) == key throughout: sb[lo..] >= key, so the upper bound
       // is the end of the run of equal keys.
       bool longer = false;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        up[q] = lo[q];
-        const uint32_t lim = nbx - lo[q] < kRunWin ? nbx : lo[q] + kRunWin;
-        for (uint32_t step = kRunWin; step; step >>= 1)
-          if (up[q] + step <= lim && !lt(kv[q], sb[up[q] + step - 1])) up[q] += step;
-        longer |= up[q] == lo[q] + kRunWin && up[q] < nbx;
+        lo[q] = off[q] / SZ;
+        uint32_t u = off[q];
+#pragma unroll
+        for (int sh = 4; sh >= 0; --sh) {
+          const uint32_t step = 1u << sh;
+          const K x = *reinterpret_cast<const K*>(sbb + u + (step - 1) * SZ);
+          u = lt(kv[q], x) ? u : u + step * SZ;
+        }
+        const uint32_t uw = u / SZ;
+        up[q] = uw < nbx ? uw : nbx;
+        longer |= uw == lo[q] + kRunWin && uw < nbx;
       }
       if (ballot(longer)) {
         // Some run fills the window: finish those from its end.
@@ -287,7 +330,7 @@ __global__ __launch_bounds__(NT * kFbSub, 8) void k_find_bounds(
   uint64_t my_sum = 0;
 #pragma unroll
   for (int j = 0; j < VT; ++j) my_sum += (c0 + j < nat) ? s_cnt[cnt_slot(c0 + j)] : 0u;
-  const uint32_t w = threadIdx.x / kWave;
+  const uint32_t w = wave_id();
   const uint64_t incl = wave_incl_scan(my_sum);
   if (lane_id() == kWave - 1) s_wave[w] = incl;
   __syncthreads();

@@ -308,7 +308,7 @@ __device__ __forceinline__ void mapfilter_tile(Filter filter, Compute compute, S
   __shared__ uint16_t s_rows[kMTile];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here

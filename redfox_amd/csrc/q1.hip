@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kQNT) void k_q1_partial(
     add_row(a, shipdate[i], flag[i] * 2u + status[i], qty[i], price[i], disc[i], tax[i], cutoff);
 
   // Block sum of each of the 36 values: a wave tree, then the waves in order.
-  const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
+  const uint32_t lane = lane_id(), w = wave_id();
 #pragma unroll
   for (int k = 0; k < kQGroups; ++k) {
 #pragma unroll

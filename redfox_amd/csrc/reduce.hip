@@ -174,7 +174,7 @@ template <class V, class Op>
 __device__ __forceinline__ Seg<V> tile_scan(const Seg<V>& mine, Seg<V>* s_wave, Seg<V>* before,
                                             bool* have_before) {
   constexpr int NW = kBNT / kWave;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid / kWave;
+  const uint32_t lane = lane_id(), wv = wave_id();
   Seg<V> incl = mine;
 #pragma unroll
   for (uint32_t d = 1; d < kWave; d <<= 1) {
@@ -228,7 +228,7 @@ template <int NT, class V, class Op>
 __device__ __forceinline__ SegE<V, Op> block_scan_excl(SegE<V, Op> incl, SegE<V, Op>* s_wave,
                                                        SegE<V, Op>* total) {
   constexpr int NW = NT / kWave;
-  const uint32_t lane = lane_id(), wv = threadIdx.x / kWave;
+  const uint32_t lane = lane_id(), wv = wave_id();
 #pragma unroll
   for (uint32_t d = 1; d < kWave; d <<= 1) {
     const SegE<V, Op> y = sege_shfl_up(incl, d);

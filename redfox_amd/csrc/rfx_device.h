@@ -181,6 +181,13 @@ __device__ __forceinline__ T wave_incl_scan(T x) {
   }
   return x;
 }
+// This thread's wave within its block.  The same for every lane, and said
+// so with readfirstlane: values derived from it (wave offsets, row bases)
+// stay in scalar registers instead of being recomputed per lane.
+__device__ __forceinline__ uint32_t wave_id() {
+  return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T x) {
 #pragma unroll
@@ -194,7 +201,7 @@ template <int NT, class T>
 __device__ __forceinline__ T block_excl_scan(T x, T* smem, T* total) {
   constexpr int NW = NT / kWave;
   const uint32_t lane = lane_id();
-  const uint32_t w = threadIdx.x / kWave;
+  const uint32_t w = wave_id();
   T incl = wave_incl_scan(x);
   if (lane == kWave - 1) smem[w] = incl;
   __syncthreads();

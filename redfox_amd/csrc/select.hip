@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kFNT) void k_select(const K* __restrict__ in, uint6
   __shared__ uint32_t s_cnt[VT * kFNW + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here

@@ -257,7 +257,7 @@ __global__ __launch_bounds__(NT) void k_set_op(
   for (int k = 1; k < VT; ++k)
     if (static_cast<uint32_t>(k) < nk) my_last = mv[k];
   const bool last_a = nk && ((amask >> (nk - 1)) & 1u);
-  const uint32_t lane = lane_id(), w = tid / kWave;
+  const uint32_t lane = lane_id(), w = wave_id();
   if (lane == kWave - 1) { s_wlast[w] = my_last; s_wlast_a[w] = last_a; }
   if (lane == 0) s_wfirst[w] = mv[0];
   bool dup = false;

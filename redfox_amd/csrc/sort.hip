@@ -216,7 +216,7 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = lane_id();
-  const uint32_t w = tid / kWave;
+  const uint32_t w = wave_id();
   const uint32_t shift = static_cast<uint32_t>(DB * pass);
   const uint32_t dmask = pass_mask<DB>(pass, key_bits);
 

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kSNT) void k_count_scan(int32_t* __restrict__ count
   __shared__ uint64_t s_scan[kSNT / kWave + 1];
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
