@@ -761,10 +761,18 @@ class Plan:
         if A.n == 0 or B.n == 0:
             return self._empty(out_types)
         strings = "str" in A.types[:k]
-        if strings and k != 1:
-            raise ValueError("a join on a string key takes that one key column")
         lb, lc, tot = self._alloc("i32", A.n), self._alloc("i32", A.n), self._buf(8)
-        if strings:
+        rtmp = []
+        if strings and k > 1:
+            # Several key columns, some of them strings: each string key
+            # column compares as its ranks over both sides' strings, and the
+            # packed integer keys join as usual (output columns come from
+            # the original relations).
+            (A2, B2), _, rtmp = self._ranked([A, B], [i for i in range(k) if A.types[i] == "str"])
+            ka, pa, kt, _, _ = self._pack(A2, range(k))
+            kb, pb, _, _, _ = self._pack(B2, range(k))
+            self.ctx.join_find_bounds(lb, lc, tot, ka, A.n, kb, B.n, 4 if kt == "u32" else 8)
+        elif strings:
             # Rows of each side in string order, their addresses in that
             # order, then the sorted search on the strings.
             ka, pa = self._string_order(A)
@@ -776,10 +784,10 @@ class Plan:
             self.ctx.join_find_bounds(lb, lc, tot, ka, A.n, kb, B.n, 4 if kt == "u32" else 8)
         total = rfx.read_u64(tot)
         if total >= 1 << 31:
-            self._release(ka, kb, pa, pb, lb, lc, tot)
+            self._release(ka, kb, pa, pb, lb, lc, tot, *rtmp)
             raise ValueError(f"join of {total} rows exceeds the int32 row index range")
         if total == 0:
-            self._release(ka, kb, pa, pb, lb, lc, tot)
+            self._release(ka, kb, pa, pb, lb, lc, tot, *rtmp)
             return self._empty(out_types)
         li, ri = self._alloc("i32", total), self._alloc("i32", total)
         self.ctx.join_main(li, ri, tot, total, lb, lc, A.n)
@@ -809,7 +817,7 @@ class Plan:
                     v, t = col(i)
                     x.emit(v, t)
         out = self._mapfilter(cols, total, build, out_types)
-        self._release(ka, kb, pa, pb, lb, lc, tot, li, ri)
+        self._release(ka, kb, pa, pb, lb, lc, tot, li, ri, *rtmp)
         # Pairs come in key order, left rows in their order within a key:
         # sorted by A's sorted columns where A needed no sort, else by the key
         # (string order is not the addresses' order: no claim).
@@ -929,11 +937,90 @@ class Plan:
         out.sorted_by = 0  # string order, not the addresses' order
         return out
 
+    def _string_ranks(self, pieces):
+        """Dense ranks, in string order, of the strings of several address
+        columns taken together: pieces = [(address buffer, rows)].  Returns
+        (a u32 rank column per piece, a representative address per rank as
+        f64 (the smallest address of the rank's strings: exact, addresses are
+        below 2^53), temporaries to release).  Equal strings get one rank
+        wherever they live, so string columns compare as their ranks."""
+        N = sum(n for _, n in pieces)
+        allv = self._alloc("u64", N)
+        off = 0
+        for buf, n in pieces:
+            if n:
+                self.ctx.copy(allv, buf, 8 * n, dst_off=8 * off)
+            off += n
+        rows, scratch = self._alloc("u32", N), self._alloc("u64", N + 1)
+        self.ctx.iota_u32(rows, N)
+        self.ctx.sort_strings_rows(allv, rows, scratch, N)
+        g = self._mapfilter([(rows, "u32"), (allv, "u64", N)], N, lambda x: (
+            x.emit(x.gather(1, x.cols[0]), "u64"), x.emit(M.to_float(x.gather(1, x.cols[0])), "f64")),
+            ["u64", "f64"])
+        saddr, saddr_f = g.cols[0][0], g.cols[1][0]
+        group, heads, ngroups = self._alloc("u32", N), self._alloc("i32", N), self._buf(8)
+        self.ctx.string_groups(saddr, N, group, heads, ngroups)
+        ncell, nout = self._cell(N), self._buf(8)
+        okeys, rep = self._alloc("u32", N), self._alloc("f64", N)
+        self.ctx.reduce_by_key_f64(okeys, rep, nout, group, saddr_f, ncell, N, 4, 1)
+        # Ranks back in row order: sort the (row, rank) pairs by row.
+        self.ctx.sort_pairs(rows, group, N, 4, max(1, int(N - 1).bit_length()), 4)
+        ranks, off = [], 0
+        for _, n in pieces:
+            r = self._alloc("u32", max(n, 1))
+            if n:
+                self.ctx.copy(r, group, 4 * n, src_off=4 * off)
+            ranks.append(r)
+            off += n
+        tmp = [allv, rows, scratch, saddr, saddr_f, group, heads, ngroups, ncell, nout, okeys]
+        return ranks, rep, tmp
+
+    def _ranked(self, rels, cols):
+        """The relations with each string column in `cols` replaced by its
+        ranks over all the relations' strings in that column (u32, string
+        order); returns (relations, {column: representative addresses},
+        temporaries)."""
+        out = [list(r.cols) for r in rels]
+        reps, tmp = {}, []
+        for i in cols:
+            ranks, rep, t = self._string_ranks([(r.cols[i][0], r.n) for r in rels])
+            for o, rk in zip(out, ranks):
+                o[i] = (rk, "u32")
+            reps[i] = rep
+            tmp += t + ranks + [rep]
+        return [Relation(o, r.n, 0) for o, r in zip(out, rels)], reps, tmp
+
     def set_operation(self, op: int, a_name: str, b_name: str) -> Relation:
-        """op: 0 union, 1 difference, 2 intersection, on whole tuples."""
+        """op: 0 union, 1 difference, 2 intersection, on whole tuples.
+        String columns take part by their strings: they are replaced by
+        their ranks in string order over both inputs, the set operation
+        runs on the integer tuples, and each result rank gets back an
+        address holding its string (the reference's Union.cu / Difference.cu
+        compare packed tuples; strings compare by content here, as the join
+        and GROUP BY on strings do)."""
         A, B = self.rel[a_name], self.rel[b_name]
         if A.types != B.types:
             raise ValueError("set operations need inputs of one type")
+        si = [i for i, t in enumerate(A.types) if t == "str"]
+        if not si:
+            return self._set_op_rel(op, A, B)
+        if A.n + B.n == 0:
+            return self._empty(A.types)
+        (A2, B2), reps, tmp = self._ranked([A, B], si)
+        R = self._set_op_rel(op, A2, B2)
+        if R.n == 0:
+            res = self._empty(A.types)
+        else:
+            cols = list(R.cols) + [(reps[i], "f64", A.n + B.n) for i in si]
+            slot = {i: len(R.cols) + k for k, i in enumerate(si)}
+            res = self._mapfilter(cols, R.n, lambda x: [
+                x.emit(M.to_int(x.gather(slot[i], x.cols[i])), "u64") if i in slot
+                else x.emit(x.cols[i], t) for i, t in enumerate(A.types)], A.types)
+        self._release(*[b for b, _ in R.cols], *tmp)
+        res.sorted_by = 0  # string order, not the addresses' order
+        return res
+
+    def _set_op_rel(self, op: int, A: Relation, B: Relation) -> Relation:
         kt, bits, fields = key_layout(A.types)
         kbytes = 4 if kt == "u32" else 8
         sets, tmp = [], []

@@ -4,6 +4,7 @@ COND, UNION, key join with reordering), and set operations, SINGLE,
 PRODUCT, MIN/MAX and a signed filter; results against numpy and Python
 sets computed from the same inputs."""
 import numpy as np
+import oracle
 import pytest
 
 from redfox_amd import datagen, raplan
@@ -220,3 +221,92 @@ def test_reference_no_op_plan_moves_a_to_b(ctx):
     f0, f1 = p.get("B")
     b = (f0.view(np.uint32).astype(np.uint64) << 32) | f1.view(np.uint32).astype(np.uint64)
     assert b.astype("<u8").tobytes() == a.tobytes()
+
+
+def _load_text_relation(rf, ctx, loaders, rows, types):
+    """rows of (bytes, value) as '|'-separated text in a buffer of their
+    own, parsed into (string address, value) columns; the buffer and a
+    host copy of it (for reading strings back) are returned too."""
+    text = b"".join(b"%s|%s|\n" % (k, str(v).encode()) for k, v in rows)
+    raw = np.frombuffer(text, dtype=np.uint8)
+    buf = rf.DeviceBuffer(len(raw) + 16)
+    buf.upload(raw)
+    cols, n, _ = loaders.parse_tbl(ctx, buf, types, "|", len(raw))
+    return cols, n, buf, buf.download(np.uint8, len(raw)).tobytes()
+
+
+def _string_reader(bufs):
+    def at(a):
+        for buf, host in bufs:
+            off = int(a) - buf.ptr
+            if 0 <= off < len(host):
+                return host[off:host.index(b"\0", off)]
+        raise AssertionError(f"address {a:#x} outside the text buffers")
+    return at
+
+
+def test_string_set_operations_plan(rf, ctx):
+    """UNION / INTERSECTION / DIFFERENCE of (string, int32) relations whose
+    strings live in two different text buffers (equal strings, different
+    addresses): tuples compare by their strings' content."""
+    from redfox_amd import loaders
+
+    rng = np.random.default_rng(71)
+    words = [b"AIR", b"MAIL", b"RAIL", b"SHIP", b"TRUCK", b"", b"FOB", b"REG AIR", b"\xc3\xa9t\xc3\xa9"]
+
+    def rows(n, lo, hi):
+        return [(words[int(w)], int(v)) for w, v in zip(rng.integers(lo, hi, n),
+                                                         rng.integers(-3, 4, n))]
+    a_rows, b_rows = rows(3000, 0, 7), rows(2500, 2, 9)
+    b = raplan.Builder("strsets")
+    for v in ("A", "B", "U", "I", "D"):
+        b.var(v, ["string"], ["int32"])
+    s = b.sequence(0)
+    s.setop("U", "union", "A", "B").setop("I", "intersection", "A", "B")
+    s.setop("D", "difference", "A", "B")
+    s.halt()
+    p = raplan.Plan(ctx, b.entry_exit(0, 0).bytes())
+    keep = []
+    for name, r in (("A", a_rows), ("B", b_rows)):
+        (addr, val), n, buf, host = _load_text_relation(rf, ctx, loaders, r, ["str", "i32"])
+        p.set_columns(name, [(addr, "str"), (val, "i32")], n)
+        keep.append((buf, host))
+    p.run()
+    at = _string_reader(keep)
+    sa, sb = set(a_rows), set(b_rows)
+    for name, want in (("U", sa | sb), ("I", sa & sb), ("D", sa - sb)):
+        addr, val = p.get(name)
+        got = [(at(x), int(y)) for x, y in zip(addr, val)]
+        assert len(got) == len(set(got)), name  # a set: each tuple once
+        assert sorted(got) == sorted(want), name
+        # in the string order of compare_string2 (signed bytes), then by value
+        assert got == sorted(got, key=lambda t: (oracle.string_key(t[0]), t[1])), name
+
+
+def test_multi_column_string_key_join_plan(rf, ctx):
+    """A join on two key columns, (string, int32), from relations in two
+    text buffers: every pair equal in both keys once."""
+    from redfox_amd import loaders
+
+    rng = np.random.default_rng(72)
+    names = [b"ALGERIA", b"ARGENTINA", b"BRAZIL", b"CANADA", b"EGYPT", b"ETHIOPIA"]
+    a_rows = [(names[int(i)], int(v)) for i, v in zip(rng.integers(0, 5, 700), rng.integers(0, 9, 700))]
+    b_rows = [(names[int(i)], int(v)) for i, v in zip(rng.integers(1, 6, 500), rng.integers(0, 9, 500))]
+    b = raplan.Builder("strjoin2")
+    b.var("A", ["string", "int32"]).var("B", ["string", "int32"])
+    b.var("J", ["string", "int32"])
+    s = b.sequence(0)
+    s.join("J", 2, "A", "B")
+    s.halt()
+    p = raplan.Plan(ctx, b.entry_exit(0, 0).bytes())
+    keep = []
+    for name, r in (("A", a_rows), ("B", b_rows)):
+        (addr, val), n, buf, host = _load_text_relation(rf, ctx, loaders, r, ["str", "i32"])
+        p.set_columns(name, [(addr, "str"), (val, "i32")], n)
+        keep.append((buf, host))
+    p.run()
+    at = _string_reader(keep)
+    addr, val = p.get("J")
+    got = sorted((at(x), int(y)) for x, y in zip(addr, val))
+    want = sorted(ka for ka in a_rows for kb in b_rows if ka == kb)
+    assert got == want

@@ -54,11 +54,12 @@ def test_select_double_ops(rf, ctx, orc, op):
     assert np.array_equal(got, orc.select(rows, 8, 0, 64, op, k))
 
 
-# Tiles hold 512 x 16 rows of up to 8 bytes and 512 x 24 rows of 16 bytes:
-# sizes around one and two tiles, and a prime size spanning many.
-@pytest.mark.parametrize("tb,n", [(8, 0), (8, 1), (8, 8191), (8, 8192), (8, 8193), (8, 16385),
-                                  (8, 1_000_003), (16, 1), (16, 12287), (16, 12288),
-                                  (16, 12289), (16, 24577), (16, 1_000_003)])
+# Tiles hold 512 x 24 rows of 8 and 16 bytes and 512 x 32 rows of narrower
+# tuples: sizes around one and two tiles, and a prime size spanning many.
+@pytest.mark.parametrize("tb,n", [(8, 0), (8, 1), (8, 12287), (8, 12288), (8, 12289),
+                                  (8, 24577), (8, 1_000_003), (16, 1), (16, 12287), (16, 12288),
+                                  (16, 12289), (16, 24577), (16, 1_000_003), (4, 16383),
+                                  (4, 16384), (4, 16385), (4, 32769), (2, 16385), (1, 16383)])
 def test_select_sizes_keep_order(rf, ctx, orc, tb, n):
     rows = rows_of(tb, n, 17)
     shift = 8 * tb - 24  # a 16-bit field below the top byte
@@ -66,8 +67,8 @@ def test_select_sizes_keep_order(rf, ctx, orc, tb, n):
     assert np.array_equal(got, orc.select(rows, tb, shift, 16, 3, 30_000))
 
 
-@pytest.mark.parametrize("tb", [8, 16])
-@pytest.mark.parametrize("n", [12288, 100_003])
+@pytest.mark.parametrize("tb", [4, 8, 16])
+@pytest.mark.parametrize("n", [12288, 16384, 100_003])
 def test_select_keeps_all_or_none(rf, ctx, orc, tb, n):
     rows = rows_of(tb, n, 18)
     shift = 8 * tb - 16
@@ -75,7 +76,7 @@ def test_select_keeps_all_or_none(rf, ctx, orc, tb, n):
         op = 3 if kept else 2
         got = gpu_select(rf, ctx, rows, tb, shift, 16, op, k)
         want = orc.select(rows, tb, shift, 16, op, k)
-        assert len(want) * 8 // tb == kept
+        assert len(want) * want.itemsize // tb == kept
         assert np.array_equal(got, want)
 
 
