@@ -850,10 +850,15 @@ class Plan:
         dom_types = [src.types[i] for i in dom]
         if src.n == 0:
             return self._empty(dom_types + [out_t])
-        if "str" in dom_types:
-            if len(dom) != 1:
-                raise ValueError("an AGG over a string domain takes that one domain column")
+        if "str" in dom_types and len(dom) == 1:
             return self._agg_string(src, dom[0], which, vcol, out_t)
+        reps, rtmp = {}, []
+        if "str" in dom_types:
+            # Several domain columns with strings among them: each string
+            # column groups by its rank in string order, and each result
+            # rank gets back an address holding its string.
+            (src,), reps, rtmp = self._ranked([src], [i for i in dom if src.types[i] == "str"])
+            dom_types = [src.types[i] for i in dom]
         kt, bits, fields = key_layout(dom_types)
         kbytes = 4 if kt == "u32" else 8
         n = src.n
@@ -883,14 +888,22 @@ class Plan:
             self.ctx.reduce_by_key_f64(okeys, ovals, nout, keys, vals, ncell, n, kbytes, op)
         groups = rfx.read_u64(nout)
 
+        slot = {d: 2 + k for k, d in enumerate(reps)}
+        rep_cols = [(reps[d], "f64", src.n) for d in reps]
+
         def unpack(x):
-            for f, t in zip(fields, dom_types):
-                x.emit(_unpack_expr(x.cols[0], f, bits), t)
+            for d, f, t in zip(dom, fields, dom_types):
+                v = _unpack_expr(x.cols[0], f, bits)
+                if d in slot:
+                    x.emit(M.to_int(x.gather(slot[d], v)), "u64")
+                else:
+                    x.emit(v, t)
             x.emit(_as(x.cols[1], out_t), out_t)
-        out = self._mapfilter([(okeys, _signed(kt)), (ovals, vt)], groups, unpack,
-                              dom_types + [out_t])
-        self._release(keys, vals, ncell, nout, okeys, ovals)
-        out.sorted_by = len(dom)
+        out_types = ["str" if d in slot else t for d, t in zip(dom, dom_types)] + [out_t]
+        out = self._mapfilter([(okeys, _signed(kt)), (ovals, vt)] + rep_cols, groups, unpack,
+                              out_types)
+        self._release(keys, vals, ncell, nout, okeys, ovals, *rtmp)
+        out.sorted_by = 0 if reps else len(dom)
         return out
 
     def _agg_string(self, src: Relation, d: int, which: str, vcol, out_t: str) -> Relation:

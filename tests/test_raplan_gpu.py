@@ -310,3 +310,38 @@ def test_multi_column_string_key_join_plan(rf, ctx):
     got = sorted((at(x), int(y)) for x, y in zip(addr, val))
     want = sorted(ka for ka in a_rows for kb in b_rows if ka == kb)
     assert got == want
+
+
+def test_group_by_string_and_int_plan(rf, ctx):
+    """AGG over two domain columns, (string, int32): COUNT and TOTAL per
+    distinct pair, the strings compared by content."""
+    from redfox_amd import loaders
+
+    rng = np.random.default_rng(73)
+    flags = [b"A", b"N", b"R", b"NR", b"\xc3\xa9"]
+    rows = [(flags[int(f)], int(s), float(v) / 8) for f, s, v in zip(
+        rng.integers(0, 5, 20_000), rng.integers(0, 3, 20_000), rng.integers(0, 8000, 20_000))]
+    text = b"".join(b"%s|%d|%r|\n" % r for r in rows)
+    raw = np.frombuffer(text, dtype=np.uint8)
+    buf = rf.DeviceBuffer(len(raw) + 16)
+    buf.upload(raw)
+    (addr, st, val), n, _ = loaders.parse_tbl(ctx, buf, ["str", "i32", "f64"], "|", len(raw))
+    at = _string_reader([(buf, buf.download(np.uint8, len(raw)).tobytes())])
+    b = raplan.Builder("groupby2")
+    b.var("L", ["string", "int32"], ["float64"])
+    b.var("C", ["string", "int32"], ["int64"]).var("T", ["string", "int32"], ["float64"])
+    s = b.sequence(0)
+    s.agg("C", "L", 3, [0, 1], "count").agg("T", "L", 3, [0, 1], "total", 2, 2)
+    s.halt()
+    p = raplan.Plan(ctx, b.entry_exit(0, 0).bytes())
+    p.set_columns("L", [(addr, "str"), (st, "i32"), (val, "f64")], n)
+    p.run()
+    groups = {}
+    for f, s_, v in rows:
+        groups.setdefault((f, s_), []).append(v)
+    for name, fn in (("C", len), ("T", sum)):
+        a, k2, v = p.get(name)
+        got = {(at(x), int(y)): z for x, y, z in zip(a, k2, v)}
+        assert len(got) == len(a) == len(groups), name
+        for key, vs in groups.items():
+            assert abs(got[key] - fn(vs)) <= 1e-12 * max(1.0, abs(fn(vs))), (name, key)
