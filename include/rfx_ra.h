@@ -81,11 +81,16 @@ int rfx_ctx_reserve(rfx_ctx* ctx, uint64_t bytes);
 uint64_t rfx_ctx_workspace_bytes(rfx_ctx* ctx);
 /* Tile shapes of the context's operators, for an autotuner: "sort.ipt"
  * (keys per thread of the onesweep tile: 8, 12 or 16, for 4/8-byte keys
- * with 0/4/8-byte values), "join.fb_tile" and "join.ex_tile" (2048 or 4096
- * merged items per find_bounds / expansion tile), "mapfilter.mode" (1: run
- * programs compiled for their types only, 2: interpret them only; the
- * default 0 compiles and interprets where compiling is not possible).
- * 0 restores the default.  Results do not depend on the choice. */
+ * with 0/4/8-byte values), "sort.digit" (radix digit bits, 8, 9 or 10, for
+ * the same shapes; 0 chooses from key_bits: 9 for 17-18 bits, 10 for
+ * 19-20, else 8), "join.fb_tile" and "join.ex_tile" (2048 or 4096 merged
+ * items per find_bounds / expansion tile), "select.vt" (rows per thread of
+ * the select pass: 8, 16, 24 or 32), "mapfilter.mode" (1: run programs
+ * compiled for their types only, 2: interpret them only; the default 0
+ * compiles and interprets where compiling is not possible).  0 restores
+ * the default.  Results do not depend on the choice.  rfx_sort_workspace
+ * covers every sort.ipt value at the default digit width; a context tuned
+ * to another width grows its own workspace on first use. */
 int rfx_ctx_tune(rfx_ctx* ctx, const char* knob, int value);
 int rfx_ctx_tuning(rfx_ctx* ctx, const char* knob, int* value);
 int rfx_ctx_synchronize(rfx_ctx* ctx);
