@@ -311,6 +311,12 @@ static int* tune_field(rfx_ctx* ctx, const char* knob, const int** allowed, int*
     *nallowed = 4;
     return &ctx->tune_sort_digit;
   }
+  if (!std::strcmp(knob, "sort.status")) {
+    static const int widths[] = {0, 64};
+    *allowed = widths;
+    *nallowed = 2;
+    return &ctx->tune_sort_status;
+  }
   if (!std::strcmp(knob, "join.fb_tile")) {
     *allowed = tiles;
     *nallowed = 3;

@@ -31,6 +31,7 @@ struct rfx_ctx {
   // Tile shapes chosen by rfx_ctx_tune (0: the built-in default).
   int tune_sort_ipt = 0;      // keys per thread of the onesweep tile
   int tune_sort_digit = 0;    // radix digit bits (0: chosen from the key bits)
+  int tune_sort_status = 0;   // 64: 64-bit look-back status words at any size
   int tune_fb_tile = 0;       // merged keys per find_bounds tile
   int tune_ex_tile = 0;       // merged items per expansion tile
   int tune_sel_vt = 0;        // rows per thread of the select pass

@@ -57,20 +57,75 @@ struct SortTune {
   static constexpr int TILE = NT * IPT;
 };
 
+// Look-back status words: 32 bits (2 flag bits, a 30-bit count) for up to
+// 2^30 - 1 rows, 64 bits (flags in bits 62-63) beyond, up to 2^31 - 1.
+template <class S>
+struct StatusWord;
+template <>
+struct StatusWord<uint32_t> {
+  static constexpr uint32_t kAgg = kFlagAgg, kIncl = kFlagIncl, kMask = kCountMask;
+};
+template <>
+struct StatusWord<uint64_t> {
+  static constexpr uint64_t kAgg = kFlagAgg64, kIncl = kFlagIncl64, kMask = kCountMask64;
+};
+
+template <class S>
+__device__ __forceinline__ void sw_store(S* p, S v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class S>
+__device__ __forceinline__ S sw_load(const S* p) {
+  return __hip_atomic_load(const_cast<S*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive prefix of column `col` of tile `tile` (stride words between
+// tiles), polling W predecessors per round trip: they are consumed in
+// order up to the first word not yet published (polled again from there)
+// or the first inclusive prefix.  Tiles before tile 0 count as an
+// inclusive prefix of 0.
+template <int W, class S>
+__device__ __forceinline__ uint32_t sort_lookback(const S* status, uint32_t tile, uint32_t stride,
+                                                  uint32_t col) {
+  using SW = StatusWord<S>;
+  uint32_t excl = 0;
+  int64_t k = static_cast<int64_t>(tile) - 1;
+  while (true) {
+    S v[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i)
+      v[i] = (k - i >= 0) ? sw_load(status + static_cast<uint64_t>(k - i) * stride + col) : SW::kIncl;
+    bool stop = false, fin = false;
+    int used = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      const S f = v[i] & ~SW::kMask;
+      const bool take = !stop && f != 0;
+      excl += take ? static_cast<uint32_t>(v[i] & SW::kMask) : 0u;
+      used += take;
+      fin |= take && f == SW::kIncl;
+      stop |= (f == 0) || (take && f == SW::kIncl);
+    }
+    if (fin) return excl;
+    k -= used;
+    if (used < W) __builtin_amdgcn_s_sleep(1);
+  }
+}
+
 struct SortLayout {
   uint32_t* hist;    // [P][bins]
   uint32_t* base;    // [P][bins] exclusive global digit offsets
   uint32_t* ctrs;    // [P][2]   tile ticket, finished tiles
   uint32_t* skip;    // [P]
   uint32_t* sel;     // [1]      0: data in caller's buffers, 1: in pong
-  uint32_t* status;  // [P][tiles][bins]
+  void* status;      // [P][tiles][bins] status words of S
   void* pong_keys;
   void* pong_vals;
   uint64_t zero_words;  // words from hist to the end of status
 };
 
 static void carve_sort(Carve& c, uint64_t n, int kb, int vb, int passes, uint64_t tiles,
-                       uint32_t bins, SortLayout* L) {
+                       uint32_t bins, SortLayout* L, bool wide = false) {
   // The zeroed block comes first so it is one contiguous range.
   const uint64_t status_words = static_cast<uint64_t>(passes) * tiles * bins;
   L->hist = c.take<uint32_t>(static_cast<uint64_t>(passes) * bins);
@@ -78,7 +133,8 @@ static void carve_sort(Carve& c, uint64_t n, int kb, int vb, int passes, uint64_
   L->ctrs = c.take<uint32_t>(static_cast<uint64_t>(passes) * 2);
   L->skip = c.take<uint32_t>(static_cast<uint64_t>(passes));
   L->sel = c.take<uint32_t>(1);
-  L->status = c.take<uint32_t>(status_words);
+  L->status = wide ? static_cast<void*>(c.take<uint64_t>(status_words))
+                   : static_cast<void*>(c.take<uint32_t>(status_words));
   L->zero_words = (c.off - (reinterpret_cast<uintptr_t>(L->hist) - c.base)) / 4;
   L->pong_keys = c.take<char>(n * kb);
   L->pong_vals = vb ? c.take<char>(n * vb) : nullptr;
@@ -180,12 +236,12 @@ __global__ __launch_bounds__(kBinsOf<DB>) void k_radix_scan(const uint32_t* __re
   if (c == n) skip[p] = 1u;  // every key has the same digit here
 }
 
-template <int KB, int VB, int NT, int IPT, int DB>
+template <int KB, int VB, int NT, int IPT, int DB, class S>
 __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
     typename Elem<KB>::T* __restrict__ k0, typename Elem<KB>::T* __restrict__ k1,
     typename Elem<VB>::T* __restrict__ v0, typename Elem<VB>::T* __restrict__ v1, uint64_t n,
     int pass, uint32_t key_bits, const uint32_t* __restrict__ base, uint32_t* __restrict__ ctrs,
-    const uint32_t* __restrict__ skip, uint32_t* __restrict__ sel, uint32_t* __restrict__ status,
+    const uint32_t* __restrict__ skip, uint32_t* __restrict__ sel, S* __restrict__ status,
     uint32_t ntiles) {
   using K = typename Elem<KB>::T;
   using V = typename Elem<VB>::T;
@@ -226,7 +282,8 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
   const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
   const uint64_t tile_base = static_cast<uint64_t>(tile) * TILE;
   const uint64_t wave_base = tile_base + static_cast<uint64_t>(w) * kWave * IPT;
-  uint32_t* st = status + (static_cast<uint64_t>(pass) * ntiles) * kBins;
+  using SW = StatusWord<S>;
+  S* st = status + (static_cast<uint64_t>(pass) * ntiles) * kBins;
 
   // Every tile but the last is full: its copy of the body has no per-key
   // bounds checks.
@@ -301,8 +358,8 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
           run += wcnt[ww][bin];
         }
         total_d[q] = run;
-        status_store(st + static_cast<uint64_t>(tile) * kBins + bin,
-                     (tile == 0 ? kFlagIncl : kFlagAgg) | run);
+        sw_store<S>(st + static_cast<uint64_t>(tile) * kBins + bin,
+                    (tile == 0 ? SW::kIncl : SW::kAgg) | static_cast<S>(run));
       }
     }
     {
@@ -351,9 +408,9 @@ __global__ __launch_bounds__(NT, RFX_SORT_MINW) void k_onesweep(
       if (bin < kBins) {
         uint32_t excl = 0;
         if (tile > 0) {
-          excl = lookback_window<RFX_SORT_LBW>(st, tile, kBins, bin);
-          status_store(st + static_cast<uint64_t>(tile) * kBins + bin,
-                       kFlagIncl | (excl + total_d[q]));
+          excl = sort_lookback<RFX_SORT_LBW>(st, tile, kBins, bin);
+          sw_store<S>(st + static_cast<uint64_t>(tile) * kBins + bin,
+                      SW::kIncl | static_cast<S>(excl + total_d[q]));
         }
         gdelta[bin] = base[pass * kBins + bin] + excl - bs[q];
       }
@@ -426,14 +483,19 @@ template <int KB, int VB>
 constexpr bool kSortTunable = (KB == 4 || KB == 8) && (VB == 0 || VB == 4 || VB == 8);
 constexpr int kSortMinIPT = 8;
 
-uint64_t sort_ws_tile(uint64_t n, int kb, int vb, int key_bits, uint64_t tile, int db) {
+uint64_t sort_ws_tile(uint64_t n, int kb, int vb, int key_bits, uint64_t tile, int db,
+                      bool wide = false) {
   const int passes = (key_bits + db - 1) / db;
   const uint64_t tiles = (n + tile - 1) / tile;
   Carve s(nullptr);
   SortLayout L;
-  carve_sort(s, n, kb, vb, passes, tiles, 1u << db, &L);
+  carve_sort(s, n, kb, vb, passes, tiles, 1u << db, &L, wide);
   return s.off;
 }
+
+// Sorts of more than 2^30 - 1 rows take 64-bit look-back status words, at
+// the default tile and 8-bit digits.
+constexpr uint64_t kSortMaxRows = (1ull << 31) - 1;
 
 // Digit width chosen for key_bits when the context does not set one.  An
 // odd number of passes leaves the data in the pong buffer and costs a copy
@@ -453,6 +515,8 @@ template <int KB, int VB>
 uint64_t sort_ws_bytes(uint64_t n, int key_bits) {
   using Tn = SortTune<KB, VB>;
   const int ipt = kSortTunable<KB, VB> && kSortMinIPT < Tn::IPT ? kSortMinIPT : Tn::IPT;
+  if (n > kCountMask)
+    return sort_ws_tile(n, KB, VB, key_bits, static_cast<uint64_t>(Tn::NT) * Tn::IPT, 8, true);
   uint64_t ws = sort_ws_tile(n, KB, VB, key_bits, static_cast<uint64_t>(Tn::NT) * ipt, 8);
   if (kSortTunable<KB, VB> && auto_digit(key_bits) != 8) {
     const int wide_ipt = Tn::IPT < 12 ? Tn::IPT : 12;
@@ -464,22 +528,23 @@ uint64_t sort_ws_bytes(uint64_t n, int key_bits) {
   return ws;
 }
 
-template <int KB, int VB, int IPT, int DB = 8>
+template <int KB, int VB, int IPT, int DB = 8, class S = uint32_t>
 int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
   using K = typename Elem<KB>::T;
   using V = typename Elem<VB>::T;
   constexpr int NT = SortTune<KB, VB>::NT;
   constexpr int TILE = NT * IPT;
+  constexpr bool kWide = sizeof(S) == 8;
   if (n < 2) return RFX_OK;
-  if (n > kCountMask) return fail(RFX_ERANGE, "sort: %llu rows exceeds 2^30-1",
-                                  static_cast<unsigned long long>(n));
+  if (n > (kWide ? kSortMaxRows : static_cast<uint64_t>(kCountMask)))
+    return fail(RFX_ERANGE, "sort: %llu rows exceeds 2^31-1", static_cast<unsigned long long>(n));
   constexpr uint32_t kBins = kBinsOf<DB>;
   const int passes = (key_bits + DB - 1) / DB;
   const uint64_t tiles = (n + TILE - 1) / TILE;
-  RFX_TRY(ensure_workspace(ctx, sort_ws_tile(n, KB, VB, key_bits, TILE, DB)));
+  RFX_TRY(ensure_workspace(ctx, sort_ws_tile(n, KB, VB, key_bits, TILE, DB, kWide)));
   Carve c(ctx->ws);
   SortLayout L;
-  carve_sort(c, n, KB, VB, passes, tiles, kBins, &L);
+  carve_sort(c, n, KB, VB, passes, tiles, kBins, &L, kWide);
   RFX_TRY(fill_u32(ctx, L.hist, L.zero_words, 0u));
 
   uint64_t hb = (n + 256 * 16 - 1) / (256 * 16);
@@ -500,10 +565,10 @@ int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
   RFX_LAUNCHED();
   for (int p = 0; p < passes; ++p) {
     RFX_PROF(ctx, "sort.onesweep");
-    k_onesweep<KB, VB, NT, IPT, DB><<<static_cast<uint32_t>(tiles), NT, 0, ctx->stream>>>(
+    k_onesweep<KB, VB, NT, IPT, DB, S><<<static_cast<uint32_t>(tiles), NT, 0, ctx->stream>>>(
         static_cast<K*>(keys), static_cast<K*>(L.pong_keys), static_cast<V*>(vals),
         static_cast<V*>(L.pong_vals), n, p, static_cast<uint32_t>(key_bits), L.base, L.ctrs,
-        L.skip, L.sel, L.status, static_cast<uint32_t>(tiles));
+        L.skip, L.sel, static_cast<S*>(L.status), static_cast<uint32_t>(tiles));
     RFX_LAUNCHED();
   }
   uint64_t cb = (n + 255) / 256;
@@ -517,6 +582,10 @@ int sort_impl(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
 
 template <int KB, int VB>
 int sort_tuned(rfx_ctx* ctx, void* keys, void* vals, uint64_t n, int key_bits) {
+  // More than 2^30 - 1 rows (or the sort.status knob at 64): 64-bit status
+  // words, default tile, 8-bit digits.
+  if (n > kCountMask || ctx->tune_sort_status == 64)
+    return sort_impl<KB, VB, SortTune<KB, VB>::IPT, 8, uint64_t>(ctx, keys, vals, n, key_bits);
   if constexpr (kSortTunable<KB, VB>) {
     // Wider digits: the default tile, or 12 keys per thread.  Knob value 0
     // takes the digit width chosen for key_bits.

@@ -180,3 +180,48 @@ def test_wider_digits_match_oracle(rf, ctx, orc, digit, ipt, kb, vb, bits):
     assert np.array_equal(gk, ok)
     if vb:
         assert np.array_equal(gv, ov)
+
+
+@pytest.mark.parametrize("kb,vb,bits", [(4, 4, 32), (4, 0, 20), (8, 4, 64), (16, 8, 100), (2, 2, 16)])
+def test_wide_status_words_match_oracle(rf, ctx, orc, kb, vb, bits):
+    """The 64-bit look-back status words that sorts of more than 2^30 - 1
+    rows take, forced on a small sort (knob sort.status = 64): keys and
+    values bit-exact against the oracle."""
+    n = 250_007
+    keys = make(kb, n, 900 + bits)
+    vals = make(vb, n, 901) if vb else None
+    try:
+        ctx.tune("sort.status", 64)
+        gk, gv = run_sort(rf, ctx, keys, vals, kb, bits, vb)
+    finally:
+        ctx.tune("sort.status", 0)
+    ok, ov = orc.sort_pairs(keys, vals, kb, bits, vb)
+    assert np.array_equal(gk, ok)
+    if vb:
+        assert np.array_equal(gv, ov)
+
+
+def test_more_than_2p30_rows_properties(rf, ctx):
+    """2^30 + 12,345 (u32 key, u32 row index) pairs, past the 32-bit status
+    words' 2^30 - 1 rows: sorted, stable and a permutation of the input."""
+    n = (1 << 30) + 12_345
+    k0 = rf.DeviceBuffer(4 * n)
+    kbuf, vbuf = rf.DeviceBuffer(4 * n), rf.DeviceBuffer(4 * n)
+    ctx.gen_u32(k0, n, datagen.BASE_SEED + 77)
+    ctx.copy(kbuf, k0, 4 * n)
+    ctx.iota_u32(vbuf, n)
+    ctx.sort_pairs(kbuf, vbuf, n, 4, 32, 4)
+    ctx.synchronize()
+    src = k0.download(np.uint32, n)
+    k0.close()
+    k = kbuf.download(np.uint32, n)
+    kbuf.close()
+    v = vbuf.download(np.uint32, n)
+    vbuf.close()
+    assert np.all(k[1:] >= k[:-1])
+    same = k[1:] == k[:-1]
+    assert np.all(v[1:][same] > v[:-1][same])
+    assert np.array_equal(src[v], k)
+    seen = np.zeros(n, dtype=bool)
+    seen[v] = True
+    assert seen.all()
