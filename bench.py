@@ -598,14 +598,31 @@ def bench_batched(rf, ctx, reps=5):
     a, b = plain.results(), br.results()
     match = all(np.array_equal(x[0], y[0]) and np.array_equal(x[2], y[2]) and
                 np.allclose(x[1], y[1], rtol=1e-12, atol=0) for x, y in zip(a, b))
+    br.graph.close()
+    # The same batching with the plain form's (unfused) operators: how much
+    # of the speedup is batching and how much is fusion.
+    bu = batched.BatchedRunner(w, fused=False)
+    bu.capture()
+    times = []
+    for _ in range(reps):
+        bu.synchronize()
+        t0 = time.perf_counter()
+        bu.run()
+        bu.synchronize()
+        times.append(time.perf_counter() - t0)
+    t_u = median(times)
+    match = match and all(np.array_equal(x[0], y[0]) and np.array_equal(x[2], y[2]) and
+                          np.allclose(x[1], y[1], rtol=1e-12, atol=0)
+                          for x, y in zip(a, bu.results()))
+    bu.graph.close()
     res = {"instances": w.n_inst, "fact_rows": w.N, "dim_rows": w.D,
            "plain_s": t_plain, "batched_ms": 1e3 * t_b,
            "qps_plain": w.n_inst / t_plain, "qps_batched": w.n_inst / t_b,
            "speedup": t_plain / t_b, "rows_per_s_batched": w.N / t_b,
+           "batched_unfused_ms": 1e3 * t_u, "speedup_unfused": t_plain / t_u,
            "input_GBps_batched": w.bytes_per_batch() / t_b / 1e9,
            "results_match": bool(match), "autotuned": tuned["choice"],
            "autotune_timings_ms": tuned["timings_ms"], "autotune_s": tuned.get("tuning_s")}
-    br.graph.close()
     return res
 
 
@@ -679,7 +696,7 @@ def summary(name, res):
         for k, v in d.items():
             if not isinstance(v, dict):
                 if k in ("ms", "core_ms", "plan_ms", "plan_graph_ms", "batched_ms", "speedup",
-                         "speedup_qps"):
+                         "speedup_qps", "batched_unfused_ms", "speedup_unfused"):
                     parts.append(f"{prefix}{k}={v:.4g}")
                 elif k == "choice":
                     parts.append(f"{prefix}{k}={v}")

@@ -171,10 +171,15 @@ def _reduce_into(c: R.Context, p: PlainRunner, b: int, t: int) -> None:
 
 
 class BatchedRunner:
-    """All instances as one plan, captured into one HIP graph."""
+    """All instances as one plan, captured into one HIP graph.  fused=False
+    runs the plain form's operators instead of the fused ones (expansion
+    and gather into 16-byte rows, two projections, then reduce-by-key and
+    count-by-key), so that the gain of batching alone can be told from
+    that of fusion."""
 
-    def __init__(self, w: Workload):
+    def __init__(self, w: Workload, fused: bool = True):
         self.w = w
+        self.fused = fused
         self.stream = R.Stream()
         self.ctx = R.Context(0, self.stream.handle)
         N, D = w.N, w.D
@@ -194,6 +199,9 @@ class BatchedRunner:
         self.ctx.synchronize()
         self.J = R.read_u64(self.tot)
         self.key2, self.val = B(4 * max(self.J, 1)), B(8 * max(self.J, 1))
+        if not fused:
+            self.out16 = B(16 * max(self.J, 1))
+            self.okeys2, self.nout2 = B(4 * cap), R.u64_cell()
 
     def _enqueue_bounds(self) -> None:
         w, c, N, D = self.w, self.ctx, self.w.N, self.w.D
@@ -209,6 +217,24 @@ class BatchedRunner:
     def _enqueue(self) -> None:
         w, c, J = self.w, self.ctx, self.J
         self._enqueue_bounds()
+        if not self.fused:
+            # The plain form's operators over all instances at once: the
+            # 16-byte row (instance-prefixed key, grp, measure), the two
+            # projections, then sum and count as two reductions.
+            c.join_expand_gather(self.out16, 16, self.ck, 4, self.mw, 8, self.dgw, 4, self.lb,
+                                 self.lc, w.N, self.tot, J, 96, 32)
+            c.project(self.key2, 4, self.out16, 16, J, [96 + FK_BITS, 0],
+                      [self.seg_bits, GRP_BITS], [GRP_BITS, 0])
+            c.project(self.val, 8, self.out16, 16, J, [32], [64], [0])
+            c.sort_pairs(self.key2, self.val, J, 4, self.seg_bits + GRP_BITS, 8)
+            L = R.lib()
+            R._check("rfx_reduce_by_key_f64", L.rfx_reduce_by_key_f64(
+                c.handle, self.okeys.ptr, self.osum.ptr, self.nout.ptr, self.key2.ptr,
+                self.val.ptr, self.tot.ptr, J, 4, 0))
+            R._check("rfx_count_by_key", L.rfx_count_by_key(
+                c.handle, self.okeys2.ptr, self.ocnt.ptr, self.nout2.ptr, self.key2.ptr,
+                self.tot.ptr, J, 4))
+            return
         # The joined row goes straight to the (instance, grp) key column and
         # the measure column: gather + project + split in one pass.
         c.join_expand_project(self.lb, self.lc, w.N, self.tot, J, self.ck, 4, self.mw, 8, self.dgw,

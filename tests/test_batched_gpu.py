@@ -71,6 +71,11 @@ def test_batched_equals_plain_equals_numpy(rf, ctx, dup):
     br.run_eager()
     br.synchronize()
     check(br.results(), want)
+    bu = batched.BatchedRunner(w, fused=False)  # the plain form's operators, batched
+    bu.capture()
+    bu.run()
+    bu.synchronize()
+    check(bu.results(), want)
 
 
 def test_zipf_foreign_keys_are_skewed(rf, ctx):
