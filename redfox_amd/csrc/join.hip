@@ -540,19 +540,44 @@ __global__ __launch_bounds__(NT) void k_expand(
   }
   __syncthreads();
 
+  // Slot of output g: how many of the tile's rows (starts s_start[1..nr],
+  // ascending) start at or before g.  Shar's fixed-step search as in
+  // find_bounds: one probe at the largest power of two P <= nr picks the
+  // window, then steps P/2 .. 1 at compile-time offsets from a byte
+  // position, and a last probe.
   const uint32_t JA = (ng + NT - 1) / NT;
-  uint32_t top = 0;
-  if (nr) top = 1u << (31 - __builtin_clz(nr));
+  const uint32_t top = nr ? 1u << (31 - __builtin_clz(nr)) : 0u;
+  constexpr int kLogNV = __builtin_ctz(NV);
+  const char* sab = reinterpret_cast<const char*>(s_start + 1);
   uint32_t slot[VT];
 #pragma unroll
   for (int j = 0; j < VT; ++j) slot[j] = 0;
-  for (uint32_t step = top; step; step >>= 1) {
+  if (top) {
+    const uint32_t pivot = s_start[top];
 #pragma unroll
     for (int j = 0; j < VT; ++j) {
-      if (static_cast<uint32_t>(j) < JA) {
-        const uint32_t g = static_cast<uint32_t>(g0) + tid + j * NT;
-        if (slot[j] + step <= nr && s_start[slot[j] + step] <= g) slot[j] += step;
+      const uint32_t g = static_cast<uint32_t>(g0) + tid + j * NT;
+      slot[j] = pivot <= g ? (nr - top) * 4u : 0u;
+    }
+#pragma unroll
+    for (int sh = kLogNV - 1; sh >= 0; --sh) {
+      const uint32_t step = 1u << sh;
+      if (step < top) {
+#pragma unroll
+        for (int j = 0; j < VT; ++j) {
+          if (static_cast<uint32_t>(j) < JA) {
+            const uint32_t g = static_cast<uint32_t>(g0) + tid + j * NT;
+            const uint32_t x = *reinterpret_cast<const uint32_t*>(sab + slot[j] + (step - 1) * 4);
+            slot[j] = x <= g ? slot[j] + step * 4 : slot[j];
+          }
+        }
       }
+    }
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      const uint32_t g = static_cast<uint32_t>(g0) + tid + j * NT;
+      const uint32_t x = *reinterpret_cast<const uint32_t*>(sab + slot[j]);
+      slot[j] = (x <= g ? slot[j] + 4 : slot[j]) / 4;
     }
   }
   static_assert(VT % kExBatch == 0, "outputs are gathered in batches");
