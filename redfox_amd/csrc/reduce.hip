@@ -79,10 +79,11 @@ struct Sizes {
   uint64_t out_b_mul;
 };
 
-__device__ __forceinline__ uint64_t rows_of(const Sizes& z) {
-  uint64_t n = *z.n_ptr / z.n_div;
+__device__ __forceinline__ uint64_t rows_of(const Sizes& z, uint64_t raw) {
+  const uint64_t n = raw / z.n_div;
   return n < z.n_cap ? n : z.n_cap;
 }
+__device__ __forceinline__ uint64_t rows_of(const Sizes& z) { return rows_of(z, *z.n_ptr); }
 
 #ifndef RFX_RBK_VT
 #define RFX_RBK_VT 24
@@ -132,16 +133,19 @@ struct TileRows {
                                        K* s_last) {
     constexpr int TILE = kBNT * VT;
     const uint32_t tid = threadIdx.x;
+    // The keys just outside the tile go out with the rows (first and last
+    // wave), not after the barrier: one round trip less before the scan.
+    const uint32_t wv = wave_id();
+    K out_prev{}, out_next{};
+    if (wv == 0 && base) out_prev = keys[base - 1];
+    if (wv == kBNT / kWave - 1 && base + TILE < n) out_next = keys[base + TILE];
     load_rows<K, VT, VEC>(keys, first, n, k);
     if constexpr (HAS_V) load_rows<V, VT, VEC>(vals, first, n, v);
     s_first[tid] = k[0];
     s_last[tid] = k[VT - 1];
     __syncthreads();
-    K prev{}, next{};
-    if (tid) prev = s_last[tid - 1];
-    else if (base) prev = keys[base - 1];
-    if (tid + 1 < kBNT) next = s_first[tid + 1];
-    else if (base + TILE < n) next = keys[base + TILE];
+    const K prev = tid ? s_last[tid - 1] : out_prev;
+    const K next = tid + 1 < kBNT ? s_first[tid + 1] : out_next;
 #pragma unroll
     for (int j = 0; j < VT; ++j) {
       const uint64_t row = first + j;
@@ -319,10 +323,12 @@ __global__ __launch_bounds__(kBNT) void k_rbk_onepass(
   // st[0]: ticket (low half); st[1]: run count, from the last tile; st[2 + t]:
   // head-count word of tile t.
   uint64_t* hst = st + 2;
+  // The row count is read while the ticket is in flight.
+  const uint64_t n_raw = *z.n_ptr;
   if (threadIdx.x == 0) s_tile = atomicAdd(reinterpret_cast<uint32_t*>(st), 1u);
   __syncthreads();
   const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
-  const uint64_t n = rows_of(z);
+  const uint64_t n = rows_of(z, n_raw);
   const uint64_t base = static_cast<uint64_t>(tile) * TILE;
   if (base >= n) return;  // uniform over the block
   const uint64_t first = base + static_cast<uint64_t>(threadIdx.x) * VT;

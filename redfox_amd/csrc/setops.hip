@@ -185,11 +185,13 @@ __global__ __launch_bounds__(NT) void k_set_op(
   const uint32_t nm = nat + nbt;
   // The neighbours just outside the tile, loaded with it: a run of equal
   // values crosses a tile edge only where one of them equals the element,
-  // so the global gallops below start only then.
-  if (tid == 0 && a0 > 0) s_edge[0] = a[a0 - 1];
-  if (tid == 1 && b0 > 0) s_edge[1] = b[b0 - 1];
-  if (tid == 2 && b1 < nb) s_edge[2] = b[b1];
-  if (tid == 3 && a1 < na) s_edge[3] = a[a1];
+  // so the global gallops below start only then.  They reach LDS after the
+  // tile's loads are issued, so that no wave waits on them alone.
+  K edge = K{};
+  if (tid == 0 && a0 > 0) edge = a[a0 - 1];
+  if (tid == 1 && b0 > 0) edge = b[b0 - 1];
+  if (tid == 2 && b1 < nb) edge = b[b1];
+  if (tid == 3 && a1 < na) edge = a[a1];
   if constexpr (sizeof(K) <= 8) {
     // Each item's address is A's or B's, chosen without a branch, so every
     // load of a full tile is in flight before the first wait (a branch per
@@ -221,6 +223,7 @@ __global__ __launch_bounds__(NT) void k_set_op(
   } else {
     for (uint32_t i = tid; i < nm; i += NT) keys[i] = i < nat ? a[a0 + i] : b[b0 + (i - nat)];
   }
+  if (tid < 4) s_edge[tid] = edge;  // slots whose neighbour does not exist are never read
   __syncthreads();
   const K* sa = keys;
   const K* sb = keys + nat;
@@ -423,13 +426,23 @@ __global__ __launch_bounds__(kSNT) void k_unique(const K* __restrict__ in, uint6
   const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);  // block-uniform: scalar from here
   const uint64_t base = static_cast<uint64_t>(tile) * kSNV;
   const uint32_t cnt = static_cast<uint32_t>((n - base) < kSNV ? (n - base) : kSNV);
-  for (uint32_t i = tid; i < cnt + 1; i += kSNT) {
-    if (i == 0) {
-      if (base > 0) keys[0] = in[base - 1];
-    } else {
-      keys[i] = in[base + i - 1];
-    }
+  // keys[0] = in[base - 1] (read only when base > 0), keys[1 + i] = in[base + i].
+  // The loads are unconditional (slots past cnt read the tile's last row)
+  // and all issued before the first wait.
+  K v[kSVT];
+#pragma unroll
+  for (int s = 0; s < kSVT; ++s) {
+    const uint32_t i = s * kSNT + tid;
+    v[s] = in[base + (i < cnt ? i : cnt - 1)];
   }
+  K before = K{};
+  if (tid == 0 && base > 0) before = in[base - 1];
+#pragma unroll
+  for (int s = 0; s < kSVT; ++s) {
+    const uint32_t i = s * kSNT + tid;
+    if (i < cnt) keys[1 + i] = v[s];
+  }
+  if (tid == 0) keys[0] = before;
   __syncthreads();
   uint32_t keepmask = 0;
 #pragma unroll
