@@ -122,27 +122,30 @@ __device__ const double kPow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  
 constexpr int kFNT = 256;            // rows per block
 constexpr int kStage = 48 * 1024;     // staged text bytes per block
 
-// Parse row r ([pos, end) of the text) into the columns; at(q) gives byte q.
-template <class At>
-__device__ __forceinline__ void parse_row(At at, uint8_t* text, uint64_t bytes, uint64_t r,
-                                          uint64_t pos, uint64_t end, uint8_t delim,
-                                          int n_fields, const TblCols& cols,
-                                          uint64_t* short_rows) {
+// Parse row r ([pos, end) of the text) into the columns; at(q) gives byte
+// q and scan(q, e) the first delimiter in [q, e) (e if none).  Each field's
+// end is found first and its value parsed from its own bytes only, as
+// splitting the row at the delimiters and then reading each field would
+// (so a space, sign, point or digit delimiter never joins two fields).
+template <class At, class Scan>
+__device__ __forceinline__ void parse_row(At at, Scan scan, uint8_t* text, uint64_t bytes,
+                                          uint64_t r, uint64_t pos, uint64_t end, int n_fields,
+                                          const TblCols& cols, uint64_t* short_rows) {
   if (end > pos && at(end - 1) == '\r') --end;
   int f = 0;
   while (f < n_fields) {
     const int t = cols.t[f];
+    const uint64_t fe = scan(pos, end);
     int64_t iv = 0;
     double dv = 0.0;
     if (t == RFX_TBL_CHAR) {
-      iv = pos < end && at(pos) != delim ? at(pos) : 0;
+      iv = pos < fe ? at(pos) : 0;
     } else if (t == RFX_TBL_DATE) {
       // YYYY-MM-DD: each '-'-separated part read as atoi reads it
       // (leading spaces, a '+', digits up to the first other byte).
       int part[3] = {0, 0, 0}, k = 0, state = 0;  // 0 leading, 1 digits, 2 done
-      for (uint64_t q = pos; q < end && k < 3; ++q) {
+      for (uint64_t q = pos; q < fe && k < 3; ++q) {
         const uint8_t c = at(q);
-        if (c == delim) break;
         if (c == '-') {
           ++k;
           state = 0;
@@ -156,12 +159,12 @@ __device__ __forceinline__ void parse_row(At at, uint8_t* text, uint64_t bytes, 
       iv = static_cast<int64_t>(date2int(part[0], part[1], part[2]));
     } else if (t != RFX_TBL_SKIP && t != RFX_TBL_STR) {
       uint64_t q = pos;
-      while (q < end && at(q) == ' ') ++q;
+      while (q < fe && at(q) == ' ') ++q;
       bool neg = false;
-      if (q < end && (at(q) == '-' || at(q) == '+')) neg = at(q++) == '-';
+      if (q < fe && (at(q) == '-' || at(q) == '+')) neg = at(q++) == '-';
       uint64_t m = 0;
       int frac = -1;  // digits after the point
-      for (; q < end; ++q) {
+      for (; q < fe; ++q) {
         const uint8_t c = at(q);
         if (c >= '0' && c <= '9') {
           m = m * 10 + (c - '0');
@@ -177,18 +180,16 @@ __device__ __forceinline__ void parse_row(At at, uint8_t* text, uint64_t bytes, 
         const double mag = static_cast<double>(m) / kPow10[frac > 0 ? (frac < 22 ? frac : 22) : 0];
         dv = neg ? -mag : mag;
       }
-    }
-    const uint64_t start = pos;
-    while (pos < end && at(pos) != delim) ++pos;
-    if (t == RFX_TBL_STR) {
+    } else if (t == RFX_TBL_STR) {
       // The field's address; its end (delimiter, '\r' or newline) becomes
       // its terminating NUL.  A field running to the end of the text has
       // nowhere to end: it reads as the null pointer (the empty string).
-      iv = pos < bytes ? static_cast<int64_t>(reinterpret_cast<uintptr_t>(text + start)) : 0;
-      if (pos < bytes) text[pos] = 0;
+      iv = fe < bytes ? static_cast<int64_t>(reinterpret_cast<uintptr_t>(text + pos)) : 0;
+      if (fe < bytes) text[fe] = 0;
     }
     tbl_store(cols, f, r, iv, dv);
     ++f;
+    pos = fe;
     if (pos >= end) break;
     ++pos;  // past the delimiter
   }
@@ -239,10 +240,33 @@ __global__ __launch_bounds__(kFNT) void k_tbl_fields(uint8_t* __restrict__ text,
   const uint64_t end = r < n_newlines ? ends[r] : bytes;
   if (staged) {
     const uint8_t* lds = reinterpret_cast<const uint8_t*>(s_text);
-    parse_row([&](uint64_t q) -> uint8_t { return lds[q - base]; }, text, bytes, r, pos, end,
-              delim, n_fields, cols, short_rows);
+    // The delimiter search reads 8 aligned bytes of LDS at a time: the
+    // lowest flagged byte of (x - 0x01..01) & ~x & 0x80..80 is x's first
+    // zero byte, x being the word XOR the delimiter in every byte.
+    const uint64_t pat = 0x0101010101010101ull * delim;
+    auto scan = [&](uint64_t q, uint64_t e) -> uint64_t {
+      uint32_t i = static_cast<uint32_t>(q - base);
+      const uint32_t ie = static_cast<uint32_t>(e - base);
+      while (i < ie && (i & 7u)) {
+        if (lds[i] == delim) return base + i;
+        ++i;
+      }
+      for (; i + 8 <= ie; i += 8) {
+        const uint64_t x = *reinterpret_cast<const uint64_t*>(lds + i) ^ pat;
+        const uint64_t z = (x - 0x0101010101010101ull) & ~x & 0x8080808080808080ull;
+        if (z) return base + i + (static_cast<uint32_t>(__builtin_ctzll(z)) >> 3);
+      }
+      while (i < ie && lds[i] != delim) ++i;
+      return base + i;
+    };
+    parse_row([&](uint64_t q) -> uint8_t { return lds[q - base]; }, scan, text, bytes, r, pos,
+              end, n_fields, cols, short_rows);
   } else {
-    parse_row([&](uint64_t q) -> uint8_t { return text[q]; }, text, bytes, r, pos, end, delim,
+    auto scan = [&](uint64_t q, uint64_t e) -> uint64_t {
+      while (q < e && text[q] != delim) ++q;
+      return q;
+    };
+    parse_row([&](uint64_t q) -> uint8_t { return text[q]; }, scan, text, bytes, r, pos, end,
               n_fields, cols, short_rows);
   }
 }

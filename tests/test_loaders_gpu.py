@@ -61,6 +61,18 @@ def test_tile_edge_newlines(rf, ctx):
     check(rf, ctx, text, ["u32"])
 
 
+@pytest.mark.parametrize("delim,text", [
+    # Empty fields between spaces: a field is its own bytes only, never the
+    # next field's digits after the delimiter.
+    (" ", b"12 34 5.5 1996-01-02 x\n 7  8\n-1 +2 .25 1998-12-31\n  \n"),
+    (".", b"1.2.3.1996-01-02.x\n7..8\n"),
+    ("+", b"1+-2+3.5+1992-01-01+y\n+5\n"),
+    ("5", b"1525352.51996-01-025x\n555\n"),
+])
+def test_delimiters_that_numbers_use(rf, ctx, delim, text):
+    check(rf, ctx, text, ["i32", "i32", "f64", "date", "char"], delim)
+
+
 def test_unaligned_text_and_capacity(rf, ctx):
     text = datagen.lineitem_tbl(3000, 5)[0]
     buf = rf.DeviceBuffer(len(text) + 16)
