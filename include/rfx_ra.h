@@ -733,8 +733,10 @@ int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, uint64_t n_r
 /* Rows of delimited text (TPC-H .tbl: fields ended by '|'; the reference's
  * TC.tbl: tab-separated) already in device memory, to typed device
  * columns (TC_data_process.cu:20-100: getline, strtok, atoi).  Rows end at
- * '\n' (a '\r' before it is dropped); a final row needs no newline.  Field
- * f of every row goes to columns[f] as types[f]:
+ * '\n' (a '\r' before it is dropped); a final row needs no newline.  A row
+ * is split at every delimiter first and each field read from its own
+ * bytes, whatever the delimiter (a space or a sign included).  Field f of
+ * every row goes to columns[f] as types[f]:
  *   RFX_TBL_SKIP  not stored            RFX_TBL_I32/I64/U32/U8  atoi-style
  *   RFX_TBL_F64   decimal digits[.digits] (mantissa / 10^k, correctly
  *                 rounded below 2^53)   RFX_TBL_DATE  YYYY-MM-DD as i64
