@@ -735,8 +735,10 @@ int rfx_mapfilter(rfx_ctx* ctx, const rfx_mf_col* cols, int n_cols, uint64_t n_r
  * columns (TC_data_process.cu:20-100: getline, strtok, atoi).  Rows end at
  * '\n' (a '\r' before it is dropped); a final row needs no newline.  A row
  * is split at every delimiter first and each field read from its own
- * bytes, whatever the delimiter (a space or a sign included).  Field f of
- * every row goes to columns[f] as types[f]:
+ * bytes, whatever the delimiter (a space or a sign included).  Two
+ * delimiters in a row make an empty field (it reads 0), where strtok
+ * would skip it; TPC-H and TC.tbl text has no empty fields, so the two
+ * agree there.  Field f of every row goes to columns[f] as types[f]:
  *   RFX_TBL_SKIP  not stored            RFX_TBL_I32/I64/U32/U8  atoi-style
  *   RFX_TBL_F64   decimal digits[.digits] (mantissa / 10^k, correctly
  *                 rounded below 2^53)   RFX_TBL_DATE  YYYY-MM-DD as i64
