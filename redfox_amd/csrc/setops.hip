@@ -19,8 +19,9 @@
 // decoupled look-back.  Runs of equal values that cross a tile edge are
 // finished by galloping in global memory, which starts only when the
 // neighbour just outside the tile is equal.  (Per-item binary searches of
-// the tile, run in lockstep, measured 2.72 ms against 1.59 ms for this on
-// a 2^27 + 2^27 u64 union: 22 LDS reads per item against about one.)
+// the tile, run in lockstep, measured 2.72 ms against 1.59 ms for the
+// first merge-path version on a 2^27 + 2^27 u64 union: 22 LDS reads per
+// item against about one.)
 #include "rfx_device.h"
 #include "rfx_internal.h"
 
