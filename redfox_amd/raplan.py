@@ -844,12 +844,45 @@ class Plan:
         dom = list(node.domains)
         which = {0: "count", 1: "total", 2: "min", 3: "max"}[a.tag]
         vcol = None if which == "count" else getattr(a, which).aggdom
-        out_t = "i64" if which == "count" else "f64"
-        if vcol is not None and not src.types[vcol].startswith("f"):
+        str_val = vcol is not None and src.types[vcol] == "str"
+        out_t = "i64" if which == "count" else "str" if str_val else "f64"
+        if str_val and which == "total":
+            raise ValueError("TOTAL runs on floating-point columns, not strings")
+        if vcol is not None and not str_val and not src.types[vcol].startswith("f"):
             raise ValueError("TOTAL/MIN/MAX run on floating-point columns (FP64 sums)")
         dom_types = [src.types[i] for i in dom]
         if src.n == 0:
             return self._empty(dom_types + [out_t])
+        if str_val:
+            # MIN/MAX of strings: the value column becomes its dense ranks in
+            # string order (exact as doubles), the ranks are reduced, and each
+            # group's rank gets back an address holding its string.
+            (ranked,), vreps, vtmp = self._ranked([src], [vcol])
+            ranked.sorted_by = min(src.sorted_by, vcol)
+            out = self._agg_columns(ranked, dom, which, vcol, "f64")
+            conv = self._rank_to_string(out, vreps[vcol], src.n)
+            self._release(*[b for b, _ in out.cols], *vtmp)
+            return conv
+        return self._agg_columns(src, dom, which, vcol, out_t)
+
+    def _rank_to_string(self, rel: Relation, rep, n_rep: int) -> Relation:
+        """rel with its last column (string ranks as f64) replaced by the
+        representative address of each rank (a STRING column)."""
+        k = len(rel.cols) - 1
+        cols = list(rel.cols) + [(rep, "f64", n_rep)]
+
+        def build(x):
+            for i, t in enumerate(rel.types[:k]):
+                x.emit(x.cols[i], t)
+            x.emit(M.to_int(x.gather(k + 1, M.to_int(x.cols[k]))), "u64")
+        out = self._mapfilter(cols, rel.n, build, rel.types[:k] + ["str"])
+        out.sorted_by = min(rel.sorted_by, k)
+        return out
+
+    def _agg_columns(self, src: Relation, dom, which: str, vcol, out_t: str) -> Relation:
+        """AGG of a non-empty relation whose value column (if any) is
+        numeric: groups by the domain columns, one aggregate per group."""
+        dom_types = [src.types[i] for i in dom]
         if "str" in dom_types and len(dom) == 1:
             return self._agg_string(src, dom[0], which, vcol, out_t)
         reps, rtmp = {}, []

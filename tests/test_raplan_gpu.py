@@ -345,3 +345,68 @@ def test_group_by_string_and_int_plan(rf, ctx):
         assert len(got) == len(a) == len(groups), name
         for key, vs in groups.items():
             assert abs(got[key] - fn(vs)) <= 1e-12 * max(1.0, abs(fn(vs))), (name, key)
+
+
+def test_min_max_of_strings_plan(rf, ctx):
+    """MIN/MAX whose value column is a STRING: the smallest/largest string
+    of each group in the string order of joins and sorts (bytes as signed
+    chars, oracle.string_key), grouped by an int32 key, by a string key and
+    by (string, int32); TOTAL of a string column is refused."""
+    from redfox_amd import loaders
+
+    rng = np.random.default_rng(97)
+    names = [b"Brand#13", b"Brand#1", b"brand#13", b"", b"\xc3\xa9t\xc3\xa9", b"Brand#13x",
+             b"zz", b"Brand#2", b"\x7f", b"A"]
+    mode = [b"AIR", b"MAIL", b"SHIP", b"\xc3\xa9"]
+    rows = [(int(k), mode[int(m)], names[int(s)]) for k, m, s in zip(
+        rng.integers(-3, 40, 30_000), rng.integers(0, 4, 30_000), rng.integers(0, 10, 30_000))]
+    text = b"".join(b"%d|%s|%s|\n" % r for r in rows)
+    raw = np.frombuffer(text, dtype=np.uint8)
+    buf = rf.DeviceBuffer(len(raw) + 16)
+    buf.upload(raw)
+    (key, md, nm), n, _ = loaders.parse_tbl(ctx, buf, ["i32", "str", "str"], "|", len(raw))
+    assert n == len(rows)
+    at = _string_reader([(buf, buf.download(np.uint8, len(raw)).tobytes())])
+    b = raplan.Builder("strminmax")
+    b.var("L", ["int32", "string"], ["string"])
+    for d in ("KLO", "KHI"):
+        b.var(d, ["int32"], ["string"])
+    for d in ("MLO", "MHI"):
+        b.var(d, ["string"], ["string"])
+    for d in ("PLO", "PHI"):
+        b.var(d, ["string", "int32"], ["string"])
+    s = b.sequence(0)
+    s.agg("KLO", "L", 3, [0], "min", 2, 2).agg("KHI", "L", 3, [0], "max", 2, 2)
+    s.agg("MLO", "L", 3, [1], "min", 2, 2).agg("MHI", "L", 3, [1], "max", 2, 2)
+    s.agg("PLO", "L", 3, [1, 0], "min", 2, 2).agg("PHI", "L", 3, [1, 0], "max", 2, 2)
+    s.halt()
+    p = raplan.Plan(ctx, b.entry_exit(0, 0).bytes())
+    p.set_columns("L", [(key, "i32"), (md, "str"), (nm, "str")], n)
+    p.run()
+    lo = lambda v: min(v, key=oracle.string_key)  # noqa: E731
+    hi = lambda v: max(v, key=oracle.string_key)  # noqa: E731
+    assert hi([b"\x7f", b"\xc3\xa9"]) == b"\x7f" and lo([b"", b"\x80"]) == b""
+    for dest, group, fn in (("KLO", lambda r: (r[0],), lo), ("KHI", lambda r: (r[0],), hi),
+                            ("MLO", lambda r: (r[1],), lo), ("MHI", lambda r: (r[1],), hi),
+                            ("PLO", lambda r: (r[1], r[0]), lo),
+                            ("PHI", lambda r: (r[1], r[0]), hi)):
+        want = {}
+        for r in rows:
+            want.setdefault(group(r), []).append(r[2])
+        cols = p.get(dest)
+        got = {}
+        for vals in zip(*[c.tolist() for c in cols]):
+            g = tuple(at(v) if isinstance(t, bytes) else int(v)
+                      for v, t in zip(vals[:-1], group(rows[0])))
+            got[g] = at(vals[-1])
+        assert len(got) == len(cols[0]) == len(want), dest
+        assert got == {g: fn(v) for g, v in want.items()}, dest
+        if dest.startswith("K"):
+            assert cols[0].tolist() == sorted(k for (k,) in want)  # int keys in key order
+    b2 = raplan.Builder("strtotal")
+    b2.var("L", ["int32", "string"], ["string"]).var("BAD", ["int32"], ["float64"])
+    b2.sequence(0).agg("BAD", "L", 3, [0], "total", 2, 2).halt()
+    p2 = raplan.Plan(ctx, b2.entry_exit(0, 0).bytes())
+    p2.set_columns("L", [(key, "i32"), (md, "str"), (nm, "str")], n)
+    with pytest.raises(ValueError, match="TOTAL"):
+        p2.run()
