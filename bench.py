@@ -90,6 +90,25 @@ def measured_traffic(path=TRAFFIC_FILE):
     return None
 
 
+def kernel_traffic(profile, kernel):
+    """HBM bytes per dispatch of one kernel from a committed rocprofv3
+    summary under profiles/ (the `traffic_bytes_per_dispatch` line of its
+    `## <kernel>` section: FETCH_SIZE and WRITE_SIZE scaled by the same
+    counters on the 1 GiB copy), or None.  The summary was taken on the
+    same workload as the bench line that quotes it."""
+    try:
+        with open(os.path.join(ROOT, "profiles", profile)) as f:
+            inside = False
+            for line in f:
+                if line.startswith("## "):
+                    inside = line[3:].startswith(kernel)
+                elif inside and line.startswith("traffic_bytes_per_dispatch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 # ---------------------------------------------------------------------------
 def bench_sort_u32(rf, ctx, steps, warmup):
     n = 1 << 28
@@ -164,7 +183,9 @@ def bench_sort_u64_zipf(rf, ctx, iters=3):
         b.close()
     return {"n": n, "ms": ms_sort, "rows_per_s": n / (ms_sort * 1e-3),
             "pass_model_GBps": 200 * n / (ms_sort * 1e-3) / 1e9,
-            "roofline": roofline(24 * n, launches, ms), "verified_sorted": ok}
+            "roofline": roofline(24 * n, launches, ms, kernel_traffic(
+                "sort_onesweep_2p27_u64_zipf.txt", "k_onesweep<8, 4, 512, 14, 8,")
+                if n == 1 << 27 else None), "verified_sorted": ok}
 
 
 def bench_join(rf, ctx, n_r, n_s, key_bits, iters):
@@ -223,10 +244,13 @@ def bench_join(rf, ctx, n_r, n_s, key_bits, iters):
     # expand+gather algorithmic bytes: lower bound + count per left row,
     # r_val per left row, s_val per output (gathered), 16 B out per output.
     eg_bytes = 12 * n_r + 4 * total + 16 * total
-    res["expand_gather_roofline"] = roofline(eg_bytes, eg[0], eg[1])
+    prof = "join_find_bounds_expand_2p24x2p26.txt" if (n_r, n_s) == (1 << 24, 1 << 26) else ""
+    res["expand_gather_roofline"] = roofline(eg_bytes, eg[0], eg[1], kernel_traffic(
+        prof, "k_expand<1, 512, 4096,") if prof else None)
     # find_bounds reads both key columns and writes a lower bound and a
     # scanned count per left row.
-    res["find_bounds_roofline"] = roofline(4 * (n_r + n_s) + 8 * n_r, fb[0], fb[1])
+    res["find_bounds_roofline"] = roofline(4 * (n_r + n_s) + 8 * n_r, fb[0], fb[1], kernel_traffic(
+        prof, "k_find_bounds<unsigned int, 256, 4096>") if prof else None)
     # The same plan (two sorts, find_bounds, expansion) captured once into
     # a HIP graph and replayed: one launch per plan.  The inputs are
     # restored before each replay, outside the timed region as above, and
@@ -295,7 +319,8 @@ def bench_q1(rf, ctx, iters=5):
         b.close()
     m = median(times)
     return {"n": n, "ms": m, "rows_per_s": n / (m * 1e-3),
-            "roofline": roofline(34 * n, launches, ms),
+            "roofline": roofline(34 * n, launches, ms,
+                                 kernel_traffic("q1_partial_60M.txt", "k_q1_partial")),
             "kept_rows": int(res[:, 5].sum())}
 
 
@@ -517,7 +542,8 @@ def bench_select(rf, ctx, iters=5):
     ctx.profile(False)
     kept = rf.read_u64(nout)
     res = {"n": n, "kept": kept, "ms": median(times[1:]), "rows_per_s": n / (median(times[1:]) * 1e-3),
-           "roofline": roofline(16 * n + 16 * kept, launches, ms)}
+           "roofline": roofline(16 * n + 16 * kept, launches, ms, kernel_traffic(
+               "select_2p27_16B.txt", "k_select<u128,") if n == 1 << 27 else None)}
     # The autotuner's choice of rows per thread for 16-, 8- and 4-byte
     # tuples of the same bytes (a 32-bit field < 2^31: half kept).
     from redfox_amd.autotune import tune
@@ -562,7 +588,9 @@ def bench_setop(rf, ctx, iters=5):
         m = rf.read_u64(nout)
         res[name] = {"out": m, "ms": median(times[1:]),
                      "rows_per_s": (na + nb) / (median(times[1:]) * 1e-3),
-                     "roofline": roofline(8 * (na + nb + m), launches, ms)}
+                     "roofline": roofline(8 * (na + nb + m), launches, ms, kernel_traffic(
+                         "set_ops_2p27_u64.txt", f"k_set_op<unsigned long, {op}, 512, 6144>")
+                         if n == 1 << 27 else None)}
     for x in (a, b, out, nout):
         x.close()
     return res
