@@ -361,7 +361,8 @@ def bench_mapfilter(rf, ctx, iters=5):
     m = median(times)
     # Every row's four columns are read once; kept rows write two doubles.
     return {"n": n, "kept": kept, "ms": m, "rows_per_s": n / (m * 1e-3), "first_call_ms": first_ms,
-            "roofline": roofline(28 * n + 16 * kept, launches, ms)}
+            "roofline": roofline(28 * n + 16 * kept, launches, ms, kernel_traffic(
+                "mapfilter_compiled_2p27_lineitem.txt", "rfx_mf_jit"))}
 
 
 PROF_TAGS = ("join.find_bounds", "join.expand", "join.expand_gather", "join.expand_project",
@@ -434,8 +435,12 @@ def bench_tbl(rf, ctx, iters=3, n_orders=1 << 20):
     out_bytes = sum(loaders.WIDTH[s] for s in spec)
     return {"bytes": len(raw), "rows": n, "ms": m, "GBps_text": len(raw) / (m * 1e-3) / 1e9,
             "rows_per_s": n / (m * 1e-3),
-            "rows_pass": roofline(len(raw) + 8 * n, rows_l, rows_ms),
-            "fields_pass": roofline(len(raw) + (out_bytes + 16) * n, fld_l, fld_ms)}
+            # parse_tbl runs the rows pass twice per call: a count (capacity
+            # 0, no row ends stored), then the parse (8 B of row end per row).
+            "rows_pass": roofline(len(raw) + 4 * n, rows_l, rows_ms, kernel_traffic(
+                "loader_stringjoin_q1plan_kernels.txt", "k_tbl_rows")),
+            "fields_pass": roofline(len(raw) + (out_bytes + 16) * n, fld_l, fld_ms, kernel_traffic(
+                "loader_stringjoin_q1plan_kernels.txt", "k_tbl_fields"))}
 
 
 def bench_join_string(rf, ctx, iters=5, nl=1 << 18, nr=1 << 22):
